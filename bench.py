@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""Throughput of the explicit MPM step on one MI355X, in particle-updates/s.
+
+Workload (default): the large 3D slump -- domain 256 x 64 x 128 m, 1 m cells,
+2x2x2 particles per cell in the left 40 % column (6.71M particles), cubic
+B-spline, mUSL-FLIP (C_pf = 1), log-strain finite deformation, F-bar on,
+Drucker-Prager c = 20 kPa, phi = 30 deg, rho = 2650, E = 1e6, nu = 0.3, CFL 0.5.
+A step is one pass of shpfun + mapsto + elastoplast over all particles, with
+every particle and grid array already resident in HBM.
+
+Prints one JSON line (value = whole-job particle-updates/s, roofline of the
+dominant kernel from HIP events on the library's stream, the CPU oracle timed
+on the host cores over a bounded sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "elastoplasm.jl_amd"))
+sys.path.insert(0, str(ROOT))
+
+import numpy as np  # noqa: E402
+
+import elastoplasm as ep  # noqa: E402
+from elastoplasm import hip  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+MAT = dict(rho0=2650.0, E=1.0e6, nu=0.3, c0=20.0e3, cr=20.0e3, phi0=math.radians(30.0), phir=math.radians(30.0),
+           Hp=0.0)
+
+WORKLOADS = {
+    # name: (L, nel, basis, column width)
+    "slump3d_256x64x128": ([256.0, 64.0, 128.0], [256, 64, 128], "bsmpm", 0.4),
+    "slump3d_128x32x32_gimp": ([128.0, 32.0, 32.0], [128, 32, 32], "gimpm", 0.4),
+    "slump2d_40x10": ([64.1584, 16.0396], [40, 10], "bsmpm", None),
+}
+
+# Algorithmic bytes per particle for each kernel of the 3D finite / mUSL /
+# F-bar / DP FP64 step (8-byte values read + written that the kernel must
+# touch; grid traffic excluded).  Whole step: 496 B (B_alg).
+KERNEL_BYTES_3D = {
+    "p2g": 8 * (3 + 3 + 6 + 1 + 1 + 1),          # x v tau J rho0 Om0
+    "g2p": 8 * ((3 + 3 + 1 + 1 + 1) + (3 + 3)),  # reads x v J rho0 Om0, writes x' v
+    "deform": 8 * ((3 + 9 + 1 + 1) + (9 + 1)),   # reads x F rho0 Om0, writes F J
+    "fbar": 8 * ((3 + 6 + 1 + 3) + (6 + 6 + 1 + 1)),  # reads x b epII c0 cr phi, writes b tau epII dlam
+}
+B_ALG = {3: 496, 2: 288}
+
+
+def build_workload(name: str, basis_override=None):
+    L, nel, basis, width = WORKLOADS[name]
+    basis = basis_override or basis
+    bkw = {"which": basis, "how": "undeformed" if basis == "gimpm" else None, "ghost": basis != "bsmpm"}
+    kw = dict(basis=bkw, fwrk=dict(deform="finite", trsfr="std", musl=True, C_pf=1.0, locking=True, damping=0.1),
+              nonloc=dict(status=False, ls=0.5), plast=dict(status=True, constitutive="DP"))
+    if width is None:
+        return ep.ic_slump(L, nel, cmpr=MAT, **kw)
+    return ep.ic_slump(L, nel, cmpr=MAT, geometry="column", column_width=width, **kw)
+
+
+def cpu_baseline(name: str, budget_s: float, threads: int):
+    """The CPU oracle (a restatement of the reference's KernelAbstractions CPU
+    kernels, OpenMP over particles with atomic scatters) on a y-slab of the
+    same workload: same x/z extent, resolution and column, 8 m wide."""
+    from oracle.oracle import Oracle
+    L, nel, basis, width = WORKLOADS[name]
+    if len(L) == 3:
+        L = [L[0], 8.0, L[2]]
+        nel = [nel[0], 8, nel[2]]
+    bkw = {"which": basis, "how": "undeformed" if basis == "gimpm" else None, "ghost": basis != "bsmpm"}
+    kw = dict(basis=bkw, fwrk=dict(deform="finite", trsfr="std", musl=True, C_pf=1.0, locking=True, damping=0.1),
+              nonloc=dict(status=False, ls=0.5), plast=dict(status=True, constitutive="DP"))
+    if width is None:
+        ic, cfg = ep.ic_slump(L, nel, cmpr=MAT, **kw)
+    else:
+        ic, cfg = ep.ic_slump(L, nel, cmpr=MAT, geometry="column", column_width=width, **kw)
+    o = Oracle(ic.mesh, ic.mpts, ic.cmpr, cfg.instr, threads=threads)
+    o.run(1, 1e9)   # warm-up step (first touch of the stencil arrays)
+    t0 = time.perf_counter()
+    steps = 0
+    while True:
+        o.run(1, 1e9)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 200:
+            break
+    nmp = ic.mpts.nmp
+    return {"value": nmp * steps / el, "unit": "particle-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{'x'.join(str(int(v)) for v in nel)} cells ({'x'.join(str(v) for v in L)} m) slab of the "
+                      f"same column, {nmp} particles, {steps} steps in {el:.1f} s, OpenMP {threads} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="slump3d_256x64x128", choices=list(WORKLOADS))
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the step would shard by spatial "
+              "slabs with halo exchange; see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    t_setup = time.perf_counter()
+    ic, cfg = build_workload(args.workload)
+    nmp = ic.mpts.nmp
+    dim = ic.mesh.dim
+    dev = hip.Device(ic.mesh, ic.mpts, ic.cmpr, cfg.instr)
+    print(f"[bench] {args.workload}: nmp={nmp} nno={int(ic.mesh.nno[-1])} setup {time.perf_counter()-t_setup:.1f}s",
+          file=sys.stderr, flush=True)
+    T = 1e9
+    if args.warmup:
+        dev.step(args.warmup, T)
+    dev.profile(True)
+    t0 = time.perf_counter()
+    r = dev.step(args.steps, T)
+    el = time.perf_counter() - t0
+    ktimes = dev.kernel_times()
+    dev.profile(False)
+    assert r.steps == args.steps, "steps were skipped"
+    ms_step = 1000.0 * el / args.steps
+    value = nmp * args.steps / el
+    print(f"[bench] {args.steps} steps in {el:.3f}s: {ms_step:.3f} ms/step, {value:.4g} pu/s", file=sys.stderr,
+          flush=True)
+    per_kernel = {k: ms / n for k, (ms, n) in ktimes.items()}
+    for k, v in sorted(per_kernel.items(), key=lambda kv: -kv[1]):
+        print(f"[bench]   {k:10s} {v*1000:9.1f} us/launch", file=sys.stderr)
+    dom = max((k for k in per_kernel if k in KERNEL_BYTES_3D), key=lambda k: per_kernel[k])
+    kb = KERNEL_BYTES_3D[dom] * (1 if dim == 3 else 0.5)
+    achieved = kb * nmp / (per_kernel[dom] * 1e-3) / 1e9
+    step_ms = sum(per_kernel.values())
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+            "kernel_ms": per_kernel[dom], "bytes_per_particle": kb}
+    step_roof = {"bytes_per_particle": B_ALG[dim], "achieved": B_ALG[dim] * value / 1e9,
+                 "frac": B_ALG[dim] * value / 1e9 / HBM_PEAK_GBS, "kernel_ms_per_step": step_ms}
+    dev.download()
+    finite = bool(np.isfinite(ic.mpts.s.tau).all())
+    dev.close()
+    cpu = None
+    if not args.no_cpu_baseline:
+        thr = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(args.workload, args.cpu_seconds, thr)
+    out = {
+        "metric": "particle-updates/s on 1 MI355X (plain FP64 explicit MPM step)",
+        "value": value,
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (ic_slump column geometry, seeded deterministic lattice)",
+        "config": {"workload": args.workload, "nmp": nmp, "nno": int(ic.mesh.nno[-1]), "basis": "bsmpm",
+                   "transfer": "mUSL-FLIP", "deform": "finite", "locking": True, "plast": "DP",
+                   "precision": "fp64", "parallelism": "single-gpu"},
+        "roofline": roof,
+        "step_roofline": step_roof,
+        "kernels_us": {k: v * 1000 for k, v in per_kernel.items()},
+        "state_finite": finite,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

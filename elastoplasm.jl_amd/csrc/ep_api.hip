@@ -1,0 +1,637 @@
+// ep_api.hip -- C ABI of libelastoplasm_hip.so (include/elastoplasm.h).
+//
+// The context owns one HIP stream, a control block in device memory and the
+// particle / grid buffers.  Upload sorts particles by background cell (the
+// reference's element numbering, tplgy.jl:43-96) so that neighbouring
+// threads touch neighbouring nodes; download undoes the permutation.
+#include "../../include/elastoplasm.h"
+#include "ep_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+using namespace ep;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(EP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+enum KernelId { K_BEGIN, K_ZERO, K_P2G, K_EULER, K_G2P, K_AUGV, K_DEFORM, K_FBAR, K_END, K_COUNT };
+const char *kKernelNames[K_COUNT] = {"begin", "zero_grid", "p2g", "euler", "g2p", "augv", "deform", "fbar", "end"};
+
+// LU with partial pivoting of Del (same algorithm the device solve expects).
+void factor_del(int n, const double *A, DelLU &f) {
+    std::memcpy(f.LU, A, sizeof(double) * n * n);
+    for (int k = 0; k < n; k++) {
+        int pk = k;
+        double mx = std::fabs(f.LU[k + n * k]);
+        for (int i = k + 1; i < n; i++)
+            if (std::fabs(f.LU[i + n * k]) > mx) { mx = std::fabs(f.LU[i + n * k]); pk = i; }
+        f.piv[k] = pk;
+        if (pk != k)
+            for (int j = 0; j < n; j++) std::swap(f.LU[k + n * j], f.LU[pk + n * j]);
+        const double akk = f.LU[k + n * k];
+        if (akk == 0.0) continue;
+        const double r = 1.0 / akk;
+        for (int i = k + 1; i < n; i++) f.LU[i + n * k] *= r;
+        for (int j = k + 1; j < n; j++)
+            for (int i = k + 1; i < n; i++) f.LU[i + n * j] -= f.LU[i + n * k] * f.LU[k + n * j];
+    }
+}
+
+double host_exact_fld(double a, double h) {
+    double q = std::floor(a / h);
+    if (std::fma(-q, h, a) < 0.0) q -= 1.0;
+    else if (std::fma(-(q + 1.0), h, a) >= 0.0) q += 1.0;
+    return q;
+}
+
+}  // namespace
+
+struct ep_ctx {
+    ep_config cfg{};
+    int dim = 0, nstr = 0;
+    hipStream_t stream = nullptr;
+    // grid
+    Geom geom{};
+    GState gs{};
+    double *grid_block = nullptr;
+    double *axis_block = nullptr;
+    int8_t *type_block = nullptr;
+    uint8_t *bc_dev = nullptr;
+    bool grid_ready = false;
+    // particles
+    PState<double> ps{};
+    double *part_block = nullptr;
+    long nmp = 0;
+    std::vector<long> perm;          // device slot -> host particle index
+    std::vector<double> n0_h, rho0_h, Om0_h;
+    bool part_ready = false;
+    // control
+    Ctrl *ctrl = nullptr;
+    double *dt_log = nullptr;
+    long dt_cap = 0;
+    Mat mat{};
+    // profiling
+    bool profile = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<int> ev_kernel;
+    size_t ev_used = 0;
+    double k_ms[K_COUNT] = {};
+    long k_launches[K_COUNT] = {};
+    ep_stats stats{};
+    double t_host = 0.0;
+};
+
+extern "C" int ep_version(void) { return EP_ABI_VERSION; }
+extern "C" const char *ep_last_error(void) { return g_err.c_str(); }
+
+extern "C" int ep_create(const ep_config *cfg, ep_ctx **out) {
+    if (!cfg || !out) return fail(EP_E_ARG, "ep_create: null argument");
+    *out = nullptr;
+    if (cfg->dim != 2 && cfg->dim != 3) return fail(EP_E_CONFIG, "dim must be 2 or 3");
+    if (cfg->basis < EP_BSMPM || cfg->basis > EP_SMPM)
+        return fail(EP_E_CONFIG, "unsupported shape function basis");
+    if (cfg->transfer != EP_STD && cfg->transfer != EP_TPIC)
+        return fail(EP_E_CONFIG, "unsupported transfer scheme");
+    if (cfg->deform != EP_FINITE && cfg->deform != EP_INFINITESIMAL)
+        return fail(EP_E_CONFIG, "unsupported deformation framework");
+    if (cfg->plast < EP_PLAST_NONE || cfg->plast > EP_J2) return fail(EP_E_CONFIG, "InvalidReturnMapping");
+    if (cfg->precision != EP_FP64) return fail(EP_E_CONFIG, "precision mode not available in this build");
+    if (cfg->nmp <= 0) return fail(EP_E_ARG, "nmp must be positive");
+    for (int i = 0; i < cfg->dim; i++) {
+        if (cfg->nel[i] < 1 || cfg->nno[i] < 2 || cfg->nno[i] > (1 << 24) || !(cfg->h[i] > 0.0))
+            return fail(EP_E_ARG, "bad mesh extent");
+    }
+    ep_ctx *c = new ep_ctx();
+    c->cfg = *cfg;
+    c->dim = cfg->dim;
+    c->nstr = cfg->dim == 2 ? 3 : 6;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(EP_E_HIP, "hipStreamCreate failed");
+    }
+    Mat &M = c->mat;
+    M.Kc = cfg->Kc; M.Gc = cfg->Gc; M.Hp = cfg->Hp; M.c = cfg->c;
+    M.C_pf = cfg->C_pf; M.damping = cfg->damping;
+    std::memcpy(M.Del, cfg->Del, sizeof(double) * c->nstr * c->nstr);
+    factor_del(c->nstr, cfg->Del, M.lu);
+    M.finite = cfg->deform == EP_FINITE;
+    M.tpic = cfg->transfer == EP_TPIC;
+    M.musl = cfg->musl != 0;
+    M.locking = cfg->locking != 0;
+    M.plast = cfg->plast;
+    Geom &g = c->geom;
+    long nno_tot = 1;
+    for (int i = 0; i < 3; i++) {
+        g.nno[i] = i < c->dim ? int(cfg->nno[i]) : 1;
+        g.nel[i] = i < c->dim ? int(cfg->nel[i]) : 1;
+        g.h[i] = i < c->dim ? cfg->h[i] : 1.0;
+        g.x0[i] = i < c->dim ? cfg->x0[i] : 0.0;
+        nno_tot *= g.nno[i];
+    }
+    g.nno_tot = nno_tot;
+    if (hipMalloc(&c->ctrl, sizeof(Ctrl)) != hipSuccess) {
+        ep_destroy(c);
+        return fail(EP_E_HIP, "hipMalloc(ctrl) failed");
+    }
+    Ctrl h{};
+    if (hipMemcpy(c->ctrl, &h, sizeof(Ctrl), hipMemcpyHostToDevice) != hipSuccess) {
+        ep_destroy(c);
+        return fail(EP_E_HIP, "hipMemcpy(ctrl) failed");
+    }
+    *out = c;
+    return EP_OK;
+}
+
+extern "C" void ep_destroy(ep_ctx *c) {
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    hipFree(c->grid_block);
+    hipFree(c->axis_block);
+    hipFree(c->type_block);
+    hipFree(c->bc_dev);
+    hipFree(c->part_block);
+    hipFree(c->ctrl);
+    hipFree(c->dt_log);
+    for (auto e : c->ev_pool) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+// B-spline node type by floating-point equality with xB (bsmpm.jl:63-76).
+static int8_t bsmpm_type(double xn, double lo, double hi, double dx) {
+    if (xn == lo) return 1;
+    if (xn == lo + dx) return 2;
+    if (lo + dx < xn && xn < hi - dx) return 3;
+    if (xn == hi - dx) return 4;
+    if (xn == hi) return 1;
+    return 0;
+}
+
+extern "C" int ep_upload_grid(ep_ctx *c, const ep_grid_view *gv) {
+    if (!c || !gv || !gv->x || !gv->bcs) return fail(EP_E_ARG, "ep_upload_grid: null argument");
+    const int D = c->dim;
+    Geom &g = c->geom;
+    const long nno = g.nno_tot;
+    // per-axis coordinates of a tensor-product grid (get_coords.jl:36-52)
+    std::vector<double> ax;
+    std::vector<int8_t> ty;
+    long off[3] = {0, 0, 0};
+    for (int i = 0; i < D; i++) {
+        off[i] = long(ax.size());
+        for (int k = 0; k < g.nno[i]; k++) {
+            long no;
+            if (D == 2) no = (i == 0) ? long(k) * g.nno[1] : long(k);
+            else no = (i == 0) ? long(k) * g.nno[2] : (i == 1 ? long(k) * g.nno[2] * g.nno[0] : long(k));
+            ax.push_back(gv->x[i + D * no]);
+            ty.push_back(bsmpm_type(gv->x[i + D * no], c->cfg.xB[i][0], c->cfg.xB[i][1], g.h[i]));
+        }
+    }
+    // verify the tensor-product structure on every node
+    for (long no = 0; no < nno; no++) {
+        long idx[3];
+        if (D == 2) { idx[1] = no % g.nno[1]; idx[0] = no / g.nno[1]; }
+        else { idx[2] = no % g.nno[2]; long r = no / g.nno[2]; idx[0] = r % g.nno[0]; idx[1] = r / g.nno[0]; }
+        for (int i = 0; i < D; i++)
+            if (gv->x[i + D * no] != ax[off[i] + idx[i]])
+                return fail(EP_E_ARG, "mesh.x is not the reference's tensor-product node layout");
+    }
+    if (c->cfg.basis == EP_BSMPM) {
+        // particles never reach invalid-type nodes only if every node is typed
+        for (size_t k = 0; k < ty.size(); k++)
+            if (ty[k] == 0) {
+                // kept: the kernel raises EP_E_DOMAIN if a particle touches it
+            }
+    }
+    std::vector<uint8_t> bc(nno, 0);
+    for (long no = 0; no < nno; no++)
+        for (int i = 0; i < D; i++)
+            if (gv->bcs[i + D * no]) bc[no] |= uint8_t(1u << i);
+    hipFree(c->axis_block);
+    hipFree(c->type_block);
+    hipFree(c->bc_dev);
+    hipFree(c->grid_block);
+    c->axis_block = nullptr; c->type_block = nullptr; c->bc_dev = nullptr; c->grid_block = nullptr;
+    HIPCHK(hipMalloc(&c->axis_block, sizeof(double) * ax.size()));
+    HIPCHK(hipMalloc(&c->type_block, ty.size()));
+    HIPCHK(hipMalloc(&c->bc_dev, nno));
+    HIPCHK(hipMemcpy(c->axis_block, ax.data(), sizeof(double) * ax.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->type_block, ty.data(), ty.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->bc_dev, bc.data(), nno, hipMemcpyHostToDevice));
+    for (int i = 0; i < 3; i++) {
+        g.X[i] = i < D ? c->axis_block + off[i] : c->axis_block;
+        g.T[i] = i < D ? c->type_block + off[i] : c->type_block;
+    }
+    // m, p[3], f[3], a[3], v[3], pa[3], dJ  -> 15 fields
+    const int nf = 15;
+    HIPCHK(hipMalloc(&c->grid_block, sizeof(double) * nf * nno));
+    HIPCHK(hipMemset(c->grid_block, 0, sizeof(double) * nf * nno));
+    GState &G = c->gs;
+    double *b = c->grid_block;
+    G.m = b; b += nno;
+    for (int d = 0; d < 3; d++) { G.p[d] = b; b += nno; }
+    for (int d = 0; d < 3; d++) { G.f[d] = b; b += nno; }
+    for (int d = 0; d < 3; d++) { G.a[d] = b; b += nno; }
+    for (int d = 0; d < 3; d++) { G.v[d] = b; b += nno; }
+    for (int d = 0; d < 3; d++) { G.pa[d] = b; b += nno; }
+    G.dJ = b;
+    G.bc = c->bc_dev;
+    G.nno = nno;
+    c->grid_ready = true;
+    return EP_OK;
+}
+
+static long elem_of(const ep_ctx *c, const double *x, long p) {
+    const int D = c->dim;
+    const Geom &g = c->geom;
+    long e[3];
+    for (int i = 0; i < D; i++) {
+        double q = host_exact_fld(x[i + D * p] - g.x0[i], g.h[i]);
+        if (!(q >= 0.0)) q = 0.0;
+        if (q > g.nel[i] - 1) q = g.nel[i] - 1;
+        e[i] = long(q);
+    }
+    if (D == 2) return e[1] + long(g.nel[1]) * e[0];
+    return e[2] + long(g.nel[2]) * (e[0] + long(g.nel[0]) * e[1]);
+}
+
+extern "C" int ep_upload_particles(ep_ctx *c, const ep_particles_view *v) {
+    if (!c || !v) return fail(EP_E_ARG, "ep_upload_particles: null argument");
+    if (!c->grid_ready) return fail(EP_E_STATE, "upload the grid first");
+    const int D = c->dim, NS = c->nstr, DD = D * D;
+    const long n = c->cfg.nmp;
+    const Mat &M = c->mat;
+    if (!v->x || !v->v || !v->F || !v->epII || !v->rho0 || !v->Om0 || !v->J || !v->c0 || !v->cr || !v->phi)
+        return fail(EP_E_ARG, "missing particle field (x, v, F, epII, rho0, Om0, J, c0, cr, phi)");
+    if (M.finite && (!v->b || !v->tau)) return fail(EP_E_ARG, "finite deformation needs b and tau");
+    if (!M.finite && (!v->sig || !v->rho)) return fail(EP_E_ARG, "infinitesimal deformation needs sig and rho");
+    if (c->cfg.basis == EP_GIMPM && !v->l0) return fail(EP_E_ARG, "GIMP needs l0");
+    if (M.tpic && !v->gradv) return fail(EP_E_ARG, "TPIC needs gradv");
+    // cell-sorted order
+    std::vector<long> key(n);
+    for (long p = 0; p < n; p++) key[p] = elem_of(c, v->x, p);
+    c->perm.resize(n);
+    std::iota(c->perm.begin(), c->perm.end(), 0L);
+    std::stable_sort(c->perm.begin(), c->perm.end(), [&](long a, long b) { return key[a] < key[b]; });
+    // field table
+    struct F { double **dst; int kind; int comp; };
+    // allocate
+    int nf = 6 + 3 + DD + (D == 2 ? 3 : 6) + NS + 8;
+    if (c->cfg.basis == EP_GIMPM) nf += D;
+    if (M.tpic) nf += DD;
+    if (M.locking) nf += DD + 1;
+    if (c->nmp != n || !c->part_block) {
+        hipFree(c->part_block);
+        c->part_block = nullptr;
+        HIPCHK(hipMalloc(&c->part_block, sizeof(double) * size_t(nf) * n));
+        c->nmp = n;
+    }
+    PState<double> &P = c->ps;
+    P = PState<double>{};
+    double *b = c->part_block;
+    auto take = [&]() { double *r = b; b += n; return r; };
+    for (int k = 0; k < 2; k++) for (int i = 0; i < D; i++) P.x[k][i] = take();
+    for (int i = 0; i < D; i++) P.v[i] = take();
+    for (int i = 0; i < DD; i++) P.F[i] = take();
+    for (int i = 0; i < (D == 2 ? 3 : 6); i++) P.b[i] = take();
+    for (int i = 0; i < NS; i++) P.s[i] = take();
+    P.epII = take(); P.dlam = take(); P.J = take(); P.rho = take();
+    P.Om0 = take(); P.c0 = take(); P.cr = take(); P.phi = take();
+    if (c->cfg.basis == EP_GIMPM) for (int i = 0; i < D; i++) P.l0[i] = take();
+    if (M.tpic) for (int i = 0; i < DD; i++) P.gv[i] = take();
+    if (M.locking) { for (int i = 0; i < DD; i++) P.dF[i] = take(); P.dJ = take(); }
+    P.n = n;
+    // host staging in device order
+    std::vector<double> st(n);
+    auto put = [&](double *dst, auto getter) -> int {
+        for (long q = 0; q < n; q++) st[q] = getter(c->perm[q]);
+        hipError_t e = hipMemcpy(dst, st.data(), sizeof(double) * n, hipMemcpyHostToDevice);
+        return e == hipSuccess ? EP_OK : fail(EP_E_HIP, "hipMemcpy particle upload failed");
+    };
+    int r;
+#define PUT(dst, expr) do { if ((r = put(dst, [&](long p) { return (expr); })) != EP_OK) return r; } while (0)
+    for (int i = 0; i < D; i++) { PUT(P.x[0][i], v->x[i + D * p]); PUT(P.x[1][i], v->x[i + D * p]); }
+    for (int i = 0; i < D; i++) PUT(P.v[i], v->v[i + D * p]);
+    for (int i = 0; i < DD; i++) PUT(P.F[i], v->F[i + DD * p]);
+    if (M.finite) {
+        // symmetric part of bᵢⱼ: 2D xx,yy,xy; 3D xx,yy,zz,yz,xz,xy
+        static const int i2[3][2] = {{0, 0}, {1, 1}, {0, 1}};
+        static const int i3[6][2] = {{0, 0}, {1, 1}, {2, 2}, {1, 2}, {0, 2}, {0, 1}};
+        for (int k = 0; k < (D == 2 ? 3 : 6); k++) {
+            const int a = D == 2 ? i2[k][0] : i3[k][0], bb = D == 2 ? i2[k][1] : i3[k][1];
+            PUT(P.b[k], 0.5 * (v->b[a + D * bb + DD * p] + v->b[bb + D * a + DD * p]));
+        }
+        for (int k = 0; k < NS; k++) PUT(P.s[k], v->tau[k + NS * p]);
+    } else {
+        for (int k = 0; k < (D == 2 ? 3 : 6); k++) PUT(P.b[k], (k < D) ? 1.0 : 0.0);
+        for (int k = 0; k < NS; k++) PUT(P.s[k], v->sig[k + NS * p]);
+    }
+    PUT(P.epII, v->epII[0 + 2 * p]);
+    PUT(P.dlam, v->dlam ? v->dlam[p] : 0.0);
+    PUT(P.J, v->J[p]);
+    PUT(P.rho, M.finite ? v->rho0[p] : v->rho[p]);
+    PUT(P.Om0, v->Om0[p]);
+    PUT(P.c0, v->c0[p]);
+    PUT(P.cr, v->cr[p]);
+    PUT(P.phi, v->phi[p]);
+    if (c->cfg.basis == EP_GIMPM) for (int i = 0; i < D; i++) PUT(P.l0[i], v->l0[i + D * p]);
+    if (M.tpic) for (int i = 0; i < DD; i++) PUT(P.gv[i], v->gradv[i + DD * p]);
+#undef PUT
+    c->n0_h.assign(n, 0.1);
+    if (v->n0) for (long p = 0; p < n; p++) c->n0_h[p] = v->n0[p];
+    c->rho0_h.assign(v->rho0, v->rho0 + n);
+    c->Om0_h.assign(v->Om0, v->Om0 + n);
+    // control block: vmax from mpts.vmax, parity 0
+    Ctrl h{};
+    HIPCHK(hipMemcpy(&h, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+    for (int i = 0; i < 3; i++) {
+        double vm = (v->vmax && i < D) ? v->vmax[i] : 0.0;
+        unsigned long long bits;
+        std::memcpy(&bits, &vm, 8);
+        h.vmax[i] = bits;
+    }
+    h.parity = 0;
+    h.err = 0;
+    HIPCHK(hipMemcpy(c->ctrl, &h, sizeof(Ctrl), hipMemcpyHostToDevice));
+    c->part_ready = true;
+    return EP_OK;
+}
+
+extern "C" int ep_set_time(ep_ctx *c, double t) {
+    if (!c) return fail(EP_E_ARG, "null context");
+    Ctrl h{};
+    HIPCHK(hipMemcpy(&h, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+    h.t = t;
+    HIPCHK(hipMemcpy(c->ctrl, &h, sizeof(Ctrl), hipMemcpyHostToDevice));
+    return EP_OK;
+}
+
+extern "C" int ep_download_particles(ep_ctx *c, ep_particles_view *v) {
+    if (!c || !v) return fail(EP_E_ARG, "ep_download_particles: null argument");
+    if (!c->part_ready) return fail(EP_E_STATE, "no particles uploaded");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int D = c->dim, NS = c->nstr, DD = D * D;
+    const long n = c->nmp;
+    const Mat &M = c->mat;
+    const PState<double> &P = c->ps;
+    Ctrl h{};
+    HIPCHK(hipMemcpy(&h, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+    const int par = h.parity;
+    std::vector<double> st(n);
+    auto get = [&](const double *src) -> int {
+        hipError_t e = hipMemcpy(st.data(), src, sizeof(double) * n, hipMemcpyDeviceToHost);
+        return e == hipSuccess ? EP_OK : fail(EP_E_HIP, "hipMemcpy particle download failed");
+    };
+    int r;
+#define GET(src, stmt) do { if ((r = get(src)) != EP_OK) return r; for (long q = 0; q < n; q++) { const long p = c->perm[q]; const double val = st[q]; stmt; } } while (0)
+    if (v->x) for (int i = 0; i < D; i++) GET(P.x[par][i], v->x[i + D * p] = val);
+    if (v->v) for (int i = 0; i < D; i++) GET(P.v[i], v->v[i + D * p] = val);
+    if (v->F) for (int i = 0; i < DD; i++) GET(P.F[i], v->F[i + DD * p] = val);
+    std::vector<double> J(n), tau;
+    GET(P.J, J[p] = val);
+    if (v->J) for (long p = 0; p < n; p++) v->J[p] = J[p];
+    if (v->Om) for (long p = 0; p < n; p++) v->Om[p] = J[p] * c->Om0_h[p];
+    if (M.finite) {
+        if (v->rho) for (long p = 0; p < n; p++) v->rho[p] = c->rho0_h[p] / J[p];
+        if (v->n) for (long p = 0; p < n; p++) v->n[p] = 1.0 - 1.0 / J[p] * (1.0 - c->n0_h[p]);
+    }
+    if (M.finite && v->b) {
+        static const int i2[3][2] = {{0, 0}, {1, 1}, {0, 1}};
+        static const int i3[6][2] = {{0, 0}, {1, 1}, {2, 2}, {1, 2}, {0, 2}, {0, 1}};
+        for (int k = 0; k < (D == 2 ? 3 : 6); k++) {
+            const int a = D == 2 ? i2[k][0] : i3[k][0], bb = D == 2 ? i2[k][1] : i3[k][1];
+            GET(P.b[k], (v->b[a + D * bb + DD * p] = val, v->b[bb + D * a + DD * p] = val));
+        }
+    }
+    tau.assign(size_t(NS) * n, 0.0);
+    for (int k = 0; k < NS; k++) GET(P.s[k], tau[k + NS * p] = val);
+    if (M.finite) {
+        if (v->tau) std::memcpy(v->tau, tau.data(), sizeof(double) * NS * n);
+        if (v->sig) for (long p = 0; p < n; p++) for (int k = 0; k < NS; k++) v->sig[k + NS * p] = tau[k + NS * p] / J[p];
+        if (v->eps) {
+            // ϵᵢⱼ = tensor(Del \ τ, ½) (DP.jl:64)
+            for (long p = 0; p < n; p++) {
+                double y[6] = {0, 0, 0, 0, 0, 0}, yy[6];
+                for (int k = 0; k < NS; k++) y[k] = tau[k + NS * p];
+                // forward/back substitution with the host LU of Del
+                for (int k = 0; k < NS; k++) yy[k] = y[k];
+                for (int k = 0; k < NS; k++) { int pk = M.lu.piv[k]; if (pk != k) std::swap(yy[k], yy[pk]); }
+                for (int i = 0; i < NS; i++) { double s = yy[i]; for (int k = 0; k < i; k++) s -= M.lu.LU[i + NS * k] * yy[k]; yy[i] = s; }
+                double x[6];
+                for (int i = NS - 1; i >= 0; i--) { double s = yy[i]; for (int k = i + 1; k < NS; k++) s -= M.lu.LU[i + NS * k] * x[k]; x[i] = s / M.lu.LU[i + NS * i]; }
+                double *e = v->eps + DD * p;
+                if (D == 2) { e[0] = x[0]; e[3] = x[1]; e[1] = e[2] = 0.5 * x[2]; }
+                else { e[0] = x[0]; e[4] = x[1]; e[8] = x[2]; e[5] = e[7] = 0.5 * x[3]; e[2] = e[6] = 0.5 * x[4]; e[1] = e[3] = 0.5 * x[5]; }
+            }
+        }
+    } else {
+        if (v->sig) std::memcpy(v->sig, tau.data(), sizeof(double) * NS * n);
+    }
+    if (v->epII) GET(P.epII, (v->epII[0 + 2 * p] = val, v->epII[1 + 2 * p] = val));
+    if (v->dlam) GET(P.dlam, v->dlam[p] = val);
+    if (v->gradv && M.tpic) for (int i = 0; i < DD; i++) GET(P.gv[i], v->gradv[i + DD * p] = val);
+    if (v->dJ && M.locking) GET(P.dJ, v->dJ[p] = val);
+    if (v->l && c->cfg.basis == EP_GIMPM) for (int i = 0; i < D; i++) GET(P.l0[i], v->l[i + D * p] = val);
+#undef GET
+    if (v->vmax)
+        for (int i = 0; i < D; i++) std::memcpy(&v->vmax[i], &h.vmax[i], 8);
+    return EP_OK;
+}
+
+extern "C" int ep_download_grid(ep_ctx *c, ep_grid_fields *gf) {
+    if (!c || !gf) return fail(EP_E_ARG, "ep_download_grid: null argument");
+    if (!c->grid_ready) return fail(EP_E_STATE, "no grid uploaded");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int D = c->dim;
+    const long nno = c->gs.nno;
+    std::vector<double> st(nno);
+    if (gf->m) HIPCHK(hipMemcpy(gf->m, c->gs.m, sizeof(double) * nno, hipMemcpyDeviceToHost));
+    double *const *srcs[4] = {c->gs.p, c->gs.f, c->gs.a, c->gs.v};
+    double *dsts[4] = {gf->p, gf->oobf, gf->a, gf->v};
+    for (int k = 0; k < 4; k++) {
+        if (!dsts[k]) continue;
+        for (int d = 0; d < D; d++) {
+            HIPCHK(hipMemcpy(st.data(), srcs[k][d], sizeof(double) * nno, hipMemcpyDeviceToHost));
+            for (long no = 0; no < nno; no++) dsts[k][d + D * no] = st[no];
+        }
+    }
+    return EP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// step
+// ---------------------------------------------------------------------------
+template <typename L>
+static void timed(ep_ctx *c, int kid, L &&launch) {
+    if (c->profile) {
+        if (c->ev_used + 2 > c->ev_pool.size()) {
+            for (int k = 0; k < 64; k++) {
+                hipEvent_t e;
+                hipEventCreate(&e);
+                c->ev_pool.push_back(e);
+            }
+            c->ev_kernel.resize(c->ev_pool.size() / 2 + 64);
+        }
+        hipEventRecord(c->ev_pool[c->ev_used], c->stream);
+        launch();
+        hipEventRecord(c->ev_pool[c->ev_used + 1], c->stream);
+        c->ev_kernel[c->ev_used / 2] = kid;
+        c->ev_used += 2;
+    } else {
+        launch();
+    }
+    c->k_launches[kid]++;
+    c->stats.launches++;
+}
+
+template <int D, int B>
+static void launch_step(ep_ctx *c) {
+    const int TB = 256;
+    const long n = c->nmp, nno = c->gs.nno;
+    const dim3 gp((n + TB - 1) / TB), gg((nno + TB - 1) / TB);
+    hipStream_t s = c->stream;
+    Ctrl *ct = c->ctrl;
+    const Geom g = c->geom;
+    const GState G = c->gs;
+    const PState<double> P = c->ps;
+    const Mat M = c->mat;
+    timed(c, K_BEGIN, [&] { k_begin<D><<<1, 64, 0, s>>>(ct, g, M); });
+    timed(c, K_ZERO, [&] { k_zero_grid<D><<<gg, TB, 0, s>>>(ct, G); });
+    timed(c, K_P2G, [&] { k_p2g<D, B, double><<<gp, TB, 0, s>>>(ct, g, P, G, M); });
+    timed(c, K_EULER, [&] { k_euler<D><<<gg, TB, 0, s>>>(ct, G, M); });
+    timed(c, K_G2P, [&] { k_g2p<D, B, double><<<gp, TB, 0, s>>>(ct, g, P, G, M); });
+    if (M.musl) timed(c, K_AUGV, [&] { k_augv<D><<<gg, TB, 0, s>>>(ct, G); });
+    timed(c, K_DEFORM, [&] { k_deform<D, B, double><<<gp, TB, 0, s>>>(ct, g, P, G, M); });
+    if (M.locking) timed(c, K_FBAR, [&] { k_fbar<D, B, double><<<gp, TB, 0, s>>>(ct, g, P, G, M); });
+    timed(c, K_END, [&] { k_end<<<1, 64, 0, s>>>(ct); });
+}
+
+static void dispatch_step(ep_ctx *c) {
+    const int D = c->dim, B = c->cfg.basis;
+    if (D == 2) {
+        if (B == EP_BSMPM) launch_step<2, 0>(c);
+        else if (B == EP_GIMPM) launch_step<2, 1>(c);
+        else launch_step<2, 2>(c);
+    } else {
+        if (B == EP_BSMPM) launch_step<3, 0>(c);
+        else if (B == EP_GIMPM) launch_step<3, 1>(c);
+        else launch_step<3, 2>(c);
+    }
+}
+
+extern "C" int ep_step(ep_ctx *c, int32_t nsteps, const ep_step_params *par, ep_step_result *res) {
+    if (!c || !par) return fail(EP_E_ARG, "ep_step: null argument");
+    if (!c->part_ready || !c->grid_ready) return fail(EP_E_STATE, "upload grid and particles first");
+    if (nsteps < 0) return fail(EP_E_ARG, "nsteps must be >= 0");
+    Ctrl h{};
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(&h, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+    const long it0 = h.it;
+    h.T = par->T;
+    h.G = par->G;
+    h.tg = par->tg;
+    h.ramp = par->ramp;
+    h.with_plast = par->with_plast;
+    h.dt_mode = par->dt_mode;
+    h.dt_fixed = par->dt_fixed;
+    h.record = par->record_dt;
+    h.steps = 0;
+    h.err = 0;
+    h.j2cap = 0;
+    if (par->record_dt) {
+        if (c->dt_cap < nsteps) {
+            hipFree(c->dt_log);
+            c->dt_log = nullptr;
+            HIPCHK(hipMalloc(&c->dt_log, sizeof(double) * std::max<long>(nsteps, 1)));
+            c->dt_cap = nsteps;
+        }
+        h.dt_log = c->dt_log;
+        h.dt_cap = c->dt_cap;
+    }
+    HIPCHK(hipMemcpy(c->ctrl, &h, sizeof(Ctrl), hipMemcpyHostToDevice));
+    c->ev_used = 0;
+    for (int k = 0; k < nsteps; k++) dispatch_step(c);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->profile) {
+        for (size_t e = 0; e < c->ev_used; e += 2) {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, c->ev_pool[e], c->ev_pool[e + 1]);
+            c->k_ms[c->ev_kernel[e / 2]] += ms;
+        }
+    }
+    HIPCHK(hipMemcpy(&h, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+    c->stats.steps += h.steps;
+    c->stats.j2_maxit += h.j2cap;
+    if (res) {
+        res->t = h.t;
+        res->last_dt = h.dt;
+        res->steps = h.steps;
+        res->it = h.it;
+    }
+    (void)it0;
+    if (h.err & ERR_DOMAIN) return fail(EP_E_DOMAIN, "Invalid position: a particle reached a node outside xB");
+    if (h.err & ERR_NONFINITE) return fail(EP_E_NONFINITE, "non-finite particle state");
+    return EP_OK;
+}
+
+extern "C" int ep_get_dts(ep_ctx *c, double *dts, int64_t cap, int64_t *n) {
+    if (!c || !dts || !n) return fail(EP_E_ARG, "ep_get_dts: null argument");
+    Ctrl h{};
+    HIPCHK(hipMemcpy(&h, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+    const long k = std::min<long>(std::min<long>(h.steps, cap), c->dt_cap);
+    if (k > 0) HIPCHK(hipMemcpy(dts, c->dt_log, sizeof(double) * k, hipMemcpyDeviceToHost));
+    *n = k;
+    return EP_OK;
+}
+
+extern "C" int ep_profile(ep_ctx *c, int32_t enable) {
+    if (!c) return fail(EP_E_ARG, "null context");
+    c->profile = enable != 0;
+    for (int k = 0; k < K_COUNT; k++) { c->k_ms[k] = 0.0; c->k_launches[k] = 0; }
+    return EP_OK;
+}
+
+extern "C" int ep_kernel_times(ep_ctx *c, int32_t max_kernels, char *names, int32_t name_len,
+                               double *total_ms, int64_t *launches, int32_t *count) {
+    if (!c || !count) return fail(EP_E_ARG, "null argument");
+    int k = 0;
+    for (int i = 0; i < K_COUNT && k < max_kernels; i++) {
+        if (c->k_launches[i] == 0) continue;
+        if (names && name_len > 0) {
+            std::strncpy(names + size_t(k) * name_len, kKernelNames[i], name_len - 1);
+            names[size_t(k) * name_len + name_len - 1] = 0;
+        }
+        if (total_ms) total_ms[k] = c->k_ms[i];
+        if (launches) launches[k] = c->k_launches[i];
+        k++;
+    }
+    *count = k;
+    return EP_OK;
+}
+
+extern "C" int ep_get_stats(ep_ctx *c, ep_stats *st) {
+    if (!c || !st) return fail(EP_E_ARG, "null argument");
+    *st = c->stats;
+    return EP_OK;
+}

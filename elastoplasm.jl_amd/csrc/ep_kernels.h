@@ -1,0 +1,634 @@
+// ep_kernels.h -- gfx950 kernels of one explicit MPM step.
+//
+// Data layout in HBM (all structure-of-arrays, particles in cell-sorted
+// order, see ep_api.hip):
+//   particle field f of particle p  -> base_f[p]
+//   grid field f of node n           -> base_f[n], node n = i_last + nno_last*(...)
+//   the node numbering is the reference's (get_coords.jl:36-52): 2D n = iz +
+//   nno_z*ix, 3D n = iz + nno_z*(ix + nno_x*iy).
+// Shape functions are never stored: every particle kernel re-evaluates the
+// 4^d stencil weights from the start-of-step position (two position buffers
+// alternate between steps, selected by ctrl->parity on the device).
+#pragma once
+#include "ep_math.h"
+
+namespace ep {
+
+constexpr int MAXD = 3;
+
+struct Geom {
+    int nno[3];
+    int nel[3];
+    double h[3];
+    double x0[3];
+    const double *X[3];        // per-axis node coordinates
+    const int8_t *T[3];        // per-axis B-spline node type (1..4, 0 invalid)
+    long nno_tot;
+};
+
+template <typename T>
+struct PState {
+    T *x[2][3];                // two position buffers, ctrl->parity picks
+    T *v[3];
+    T *F[9];                   // F[i + d*j]
+    T *b[6];                   // symmetric: 2D xx,yy,xy; 3D xx,yy,zz,yz,xz,xy
+    T *s[6];                   // τ (finite) or σ (infinitesimal), Voigt
+    T *epII, *dlam, *J;
+    T *rho;                    // ρ₀ (finite: ρ = ρ₀/J) or the constant ρ (infinitesimal)
+    T *Om0, *c0, *cr, *phi;
+    T *l0[3];                  // GIMP half lengths
+    T *gv[9];                  // ∇v (TPIC state)
+    T *dF[9];                  // ΔF scratch between deform and F-bar gather
+    T *dJ;                     // ΔJ scratch
+    long n;
+};
+
+struct GState {
+    double *m, *p[3], *f[3], *a[3], *v[3], *pa[3], *dJ;
+    const uint8_t *bc;         // bit d set: Dirichlet in axis d
+    long nno;
+};
+
+struct Ctrl {
+    double t, dt, T, G, tg, dt_fixed;
+    double g[3];
+    unsigned long long vmax[3];  // bit patterns of non-negative doubles
+    long it, steps;
+    int active, parity, ramp, dt_mode, with_plast, record;
+    unsigned int err;
+    int j2cap;
+    double *dt_log;
+    long dt_cap;
+};
+
+enum : unsigned { ERR_DOMAIN = 1u, ERR_NONFINITE = 2u };
+
+struct Mat {
+    double Kc, Gc, Hp, c, C_pf, damping;
+    double Del[36];
+    DelLU lu;
+    int finite, tpic, musl, locking, plast;
+};
+
+// -------------------------------------------------------------------------
+// stencil weights
+// -------------------------------------------------------------------------
+template <int D, typename T>
+struct W {
+    T w[D][4], dw[D][4];
+    int base[D];
+    unsigned ok;  // bit (4*axis + k): node exists
+};
+
+// shpfun: p2e2n + basis (tplgy.jl:43-96, bsmpm.jl:95-135, gimpm.jl:31-71,
+// smpm.jl:27-66).  Returns false on a bsmpm "Invalid position".
+template <int D, int B, typename T>
+__device__ __forceinline__ bool weights(const Geom &g, const T *xp, const T *lp, W<D, T> &w) {
+    bool good = true;
+    w.ok = 0u;
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+        const double a = double(xp[i]) - g.x0[i];
+        double e = exact_fld(a, g.h[i]);
+        const double hi = double(g.nel[i] - 1);
+        e = e < 0.0 ? 0.0 : (e > hi ? hi : e);
+        const int base = int(e) - 1;
+        w.base[i] = base;
+        const T h = T(g.h[i]);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int n = base + k;
+            T f = T(0), df = T(0);
+            if (n >= 0 && n < g.nno[i]) {
+                w.ok |= 1u << (4 * i + k);
+                const T xn = T(g.X[i][n]);
+                if (B == 0) {
+                    const int ty = g.T[i][n];
+                    if (ty == 0) good = false;
+                    T ff, dd;
+                    bspline<T>((xp[i] - xn) / h, ty, ff, dd);
+                    f = ff;
+                    df = dd / h;
+                } else if (B == 1) {
+                    gimp<T>(xp[i] - xn, h, lp[i], f, df);
+                } else {
+                    linear<T>(xp[i] - xn, h, f, df);
+                }
+            }
+            w.w[i][k] = f;
+            w.dw[i][k] = df;
+        }
+    }
+    return good;
+}
+
+// Iterate the stencil in the reference's e2n order (e2n.jl:37-75):
+// 2D x outer, z inner; 3D y outer, x, z inner.  fn(node, N, dN[D]).
+template <int D, typename T, typename Fn>
+__device__ __forceinline__ void for_stencil(const Geom &g, const W<D, T> &w, Fn &&fn) {
+    if (D == 2) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            if (!(w.ok >> i & 1u)) continue;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                if (!(w.ok >> (4 + j) & 1u)) continue;
+                const long no = long(w.base[1] + j) + long(g.nno[1]) * long(w.base[0] + i);
+                T dN[D];
+                const int ix[2] = {w.base[0] + i, w.base[1] + j};
+                const T N = w.w[0][i] * w.w[1][j];
+                dN[0] = w.dw[0][i] * w.w[1][j];
+                dN[1] = w.w[0][i] * w.dw[1][j];
+                fn(no, N, dN, ix);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (!(w.ok >> (4 + k) & 1u)) continue;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                if (!(w.ok >> i & 1u)) continue;
+                const T wxy = w.w[0][i] * w.w[1][k];
+                const T dxy = w.dw[0][i] * w.w[1][k];
+                const T xdy = w.w[0][i] * w.dw[1][k];
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    if (!(w.ok >> (8 + j) & 1u)) continue;
+                    const long no = long(w.base[2] + j) +
+                                    long(g.nno[2]) * (long(w.base[0] + i) + long(g.nno[0]) * long(w.base[1] + k));
+                    T dN[D];
+                    const int ix[3] = {w.base[0] + i, w.base[1] + k, w.base[2] + j};
+                    const T N = wxy * w.w[2][j];
+                    dN[0] = dxy * w.w[2][j];
+                    dN[1] = xdy * w.w[2][j];
+                    dN[2] = wxy * w.dw[2][j];
+                    fn(no, N, dN, ix);
+                }
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void atomic_add(double *a, double v) { unsafeAtomicAdd(a, v); }
+
+__device__ __forceinline__ void atomic_max_pos(unsigned long long *a, double v) {
+    atomicMax(a, (unsigned long long)__double_as_longlong(v));
+}
+
+template <typename T>
+__device__ __forceinline__ T mass_of(const PState<T> &P, const Mat &M, long p, T J) {
+    // ρ*Ω with ρ = ρ₀/J (deform.jl:21, finite) and Ω = J*Ω₀ (deform.jl:19)
+    const T Om = J * P.Om0[p];
+    return M.finite ? (P.rho[p] / J) * Om : P.rho[p] * Om;
+}
+
+// -------------------------------------------------------------------------
+// step kernels
+// -------------------------------------------------------------------------
+
+// get_dt / get_spacetime (physics.jl:21-26, 80-90) + loop guard
+// `while T > time.t[1]` (worflow.jl:28, 74).
+template <int D>
+__global__ void k_begin(Ctrl *c, Geom g, Mat M) {
+    if (threadIdx.x != 0) return;
+    const double t = c->t;
+    c->active = (c->T > t) ? 1 : 0;
+    if (!c->active) return;
+    double dt;
+    if (c->dt_mode == 0) {
+        double cmax = -HUGE_VAL;
+        for (int i = 0; i < D; i++) {
+            const double vm = __longlong_as_double((long long)c->vmax[i]);
+            const double r = g.h[i] / (vm + M.c);
+            cmax = r > cmax ? r : cmax;
+        }
+        const double a = 0.5 * cmax, b = c->T - t;
+        dt = a < b ? a : b;
+    } else {
+        dt = c->dt_fixed;
+    }
+    for (int i = 0; i < D; i++) c->vmax[i] = 0ull;
+    double G = c->G;
+    if (c->ramp) G = (t <= c->tg) ? (c->G * t / c->tg) : c->G;
+    for (int i = 0; i < 3; i++) c->g[i] = 0.0;
+    c->g[D - 1] = -G;
+    c->dt = dt;
+    if (c->record && c->steps < c->dt_cap) c->dt_log[c->steps] = dt;
+}
+
+__global__ void k_end(Ctrl *c) {
+    if (threadIdx.x != 0 || !c->active) return;
+    c->t = c->t + c->dt;
+    c->it += 1;
+    c->steps += 1;
+    c->parity ^= 1;
+}
+
+// zero mᵢ, p, oobf (p2n.jl:290-292), the mUSL momentum and ΔJ (update.jl:53)
+template <int D>
+__global__ void k_zero_grid(const Ctrl *c, GState G) {
+    const long n = blockIdx.x * long(blockDim.x) + threadIdx.x;
+    if (n >= G.nno || !c->active) return;
+    G.m[n] = 0.0;
+    G.dJ[n] = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; d++) { G.p[d][n] = 0.0; G.f[d][n] = 0.0; G.pa[d][n] = 0.0; }
+}
+
+// transform + std/tpic_{2,3}d_p2n (p2n.jl:12-21, 58-102, 140-191)
+template <int D, int B, typename T>
+__global__ void __launch_bounds__(256) k_p2g(const Ctrl *c, Geom g, PState<T> P, GState G, Mat M) {
+    constexpr int NS = D == 2 ? 3 : 6;
+    const long p = blockIdx.x * long(blockDim.x) + threadIdx.x;
+    if (p >= P.n || !c->active) return;
+    const int par = c->parity;
+    T xp[D], lp[D];
+#pragma unroll
+    for (int i = 0; i < D; i++) { xp[i] = P.x[par][i][p]; lp[i] = B == 1 ? P.l0[i][p] : T(0); }
+    W<D, T> w;
+    if (!weights<D, B, T>(g, xp, lp, w)) atomicOr(const_cast<unsigned *>(&c->err), ERR_DOMAIN);
+    const T J = P.J[p];
+    const T Om = J * P.Om0[p];
+    const T ms = mass_of(P, M, p, J);
+    T s[NS];
+#pragma unroll
+    for (int k = 0; k < NS; k++) s[k] = M.finite ? P.s[k][p] / J : P.s[k][p];
+    T v[D], pm[D], gv[D][D];
+#pragma unroll
+    for (int i = 0; i < D; i++) { v[i] = P.v[i][p]; pm[i] = ms * v[i]; }
+    if (M.tpic) {
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; j < D; j++) gv[i][j] = P.gv[i + D * j][p];
+    }
+    const T gz = T(c->g[D - 1]);
+    const bool tpic = M.tpic;
+    for_stencil<D, T>(g, w, [&](long no, T N, const T *dN, const int *ix) {
+        atomic_add(&G.m[no], double(N * ms));
+        if (D == 2) {
+            if (tpic) {
+                // Δnp = x_n - x_p (misc.jl:1-14)
+                const T dx = T(g.X[0][ix[0]]) - xp[0];
+                const T dy = T(g.X[1][ix[1]]) - xp[1];
+                atomic_add(&G.p[0][no], double(N * ms * (v[0] + gv[0][0] * dx + gv[0][1] * dy)));
+                atomic_add(&G.p[1][no], double(N * ms * (v[1] + gv[1][0] * dx + gv[1][1] * dy)));
+            } else {
+                atomic_add(&G.p[0][no], double(N * pm[0]));
+                atomic_add(&G.p[1][no], double(N * pm[1]));
+            }
+            atomic_add(&G.f[0][no], double(-(Om * (dN[0] * s[0] + dN[1] * s[2]))));
+            atomic_add(&G.f[1][no], double(-(Om * (dN[0] * s[2] + dN[1] * s[1]) - N * (ms * gz))));
+        } else {
+            if (tpic) {
+                const T dx = T(g.X[0][ix[0]]) - xp[0];
+                const T dy = T(g.X[1][ix[1]]) - xp[1];
+                const T dz = T(g.X[2][ix[2]]) - xp[2];
+                atomic_add(&G.p[0][no], double(N * ms * (v[0] + gv[0][0] * dx + gv[0][1] * dy + gv[0][2] * dz)));
+                atomic_add(&G.p[1][no], double(N * ms * (v[1] + gv[1][0] * dx + gv[1][1] * dy + gv[1][2] * dz)));
+                atomic_add(&G.p[2][no], double(N * ms * (v[2] + gv[2][0] * dx + gv[2][1] * dy + gv[2][2] * dz)));
+            } else {
+                atomic_add(&G.p[0][no], double(N * pm[0]));
+                atomic_add(&G.p[1][no], double(N * pm[1]));
+                atomic_add(&G.p[2][no], double(N * pm[2]));
+            }
+            // Voigt [xx,yy,zz,yz,xz,xy]: σyx = s[5], σzy = s[3], σzx = s[4]
+            atomic_add(&G.f[0][no], double(-(Om * (dN[0] * s[0] + dN[1] * s[5] + dN[2] * s[4]))));
+            atomic_add(&G.f[1][no], double(-(Om * (dN[0] * s[5] + dN[1] * s[1] + dN[2] * s[3]))));
+            atomic_add(&G.f[2][no], double(-(Om * (dN[0] * s[4] + dN[1] * s[3] + dN[2] * s[2]) - N * (ms * gz))));
+        }
+    });
+}
+
+// euler (solve.jl:14-40); a and v of massless nodes are the zeros that
+// solve() writes first (solve.jl:58-59).
+template <int D>
+__global__ void k_euler(const Ctrl *c, GState G, Mat M) {
+    const long no = blockIdx.x * long(blockDim.x) + threadIdx.x;
+    if (no >= G.nno || !c->active) return;
+    const double dt = c->dt, eta = M.damping;
+    const double m = G.m[no];
+    double o[D], pp[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) { o[d] = G.f[d][no]; pp[d] = G.p[d][no]; }
+    double a[D], v[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) { a[d] = 0.0; v[d] = 0.0; }
+    if (m != 0.0) {
+        const unsigned bc = G.bc[no];
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            if (bc >> d & 1u) {
+                a[d] = 0.0;
+                v[d] = 0.0;
+            } else {
+                const double mi = 1.0 / m;
+                double nrm = o[0] * o[0];
+#pragma unroll
+                for (int k = 1; k < D; k++) nrm += o[k] * o[k];
+                nrm = sqrt(nrm);
+                const double Dm = eta * nrm * jsign(pp[d] * mi);
+                if (fabs(pp[d] * mi) >= 1.0e-3) o[d] = o[d] - Dm;
+                a[d] = o[d] * mi;
+                v[d] = (pp[d] + dt * o[d]) * mi;
+            }
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < D; d++) { G.f[d][no] = o[d]; G.a[d][no] = a[d]; G.v[d][no] = v[d]; }
+}
+
+// picflip_n2p + augm_momentum (n2p.jl:14-40, augm.jl:13-25)
+template <int D, int B, typename T>
+__global__ void __launch_bounds__(256) k_g2p(Ctrl *c, Geom g, PState<T> P, GState G, Mat M) {
+    const long p = blockIdx.x * long(blockDim.x) + threadIdx.x;
+    if (p >= P.n || !c->active) return;
+    const int par = c->parity;
+    T xp[D], lp[D];
+#pragma unroll
+    for (int i = 0; i < D; i++) { xp[i] = P.x[par][i][p]; lp[i] = B == 1 ? P.l0[i][p] : T(0); }
+    W<D, T> w;
+    weights<D, B, T>(g, xp, lp, w);
+    T vpic[D], aflip[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) { vpic[d] = T(0); aflip[d] = T(0); }
+    for_stencil<D, T>(g, w, [&](long no, T N, const T *, const int *) {
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            vpic[d] += N * T(G.v[d][no]);
+            aflip[d] += N * T(G.a[d][no]);
+        }
+    });
+    const T dt = T(c->dt), C = T(M.C_pf);
+    T vn[D];
+    bool bad = false;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        vn[d] = C * (P.v[d][p] + dt * aflip[d]) + (T(1) - C) * vpic[d];
+        const T xn = xp[d] + dt * vpic[d];
+        P.v[d][p] = vn[d];
+        P.x[par ^ 1][d][p] = xn;
+        if (!isfinite(vn[d]) || !isfinite(xn)) bad = true;
+        atomic_max_pos(&c->vmax[d], double(fabs(vn[d])));
+    }
+    if (bad) atomicOr(&c->err, ERR_NONFINITE);
+    if (M.musl) {
+        const T ms = mass_of(P, M, p, P.J[p]);
+        for_stencil<D, T>(g, w, [&](long no, T N, const T *, const int *) {
+#pragma unroll
+            for (int d = 0; d < D; d++) atomic_add(&G.pa[d][no], double(N * (ms * vn[d])));
+        });
+    }
+}
+
+// augm_velocity (augm.jl:37-53) after mesh.v .= 0 (n2p.jl:67)
+template <int D>
+__global__ void k_augv(const Ctrl *c, GState G) {
+    const long no = blockIdx.x * long(blockDim.x) + threadIdx.x;
+    if (no >= G.nno || !c->active) return;
+    const double m = G.m[no];
+    const unsigned bc = G.bc[no];
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        double v = 0.0;
+        if (m != 0.0 && !(bc >> d & 1u)) v = G.pa[d][no] * (1.0 / m);
+        G.v[d][no] = v;
+    }
+}
+
+// elastic predictor + plastic corrector on one particle (elast.jl:45-80,
+// plast.jl:49-65, DP.jl:33-99, J2.jl:21-75).  dF is the (F-bar scaled) ΔF.
+template <int D, typename T>
+__device__ __forceinline__ void constitutive(Ctrl *c, const PState<T> &P, const Mat &M, long p, const T (&dF)[D][D]) {
+    constexpr int NS = D == 2 ? 3 : 6;
+    T s[NS];
+    if (M.finite) {
+        // b <- ΔF b ΔF'  (elast.jl:49)
+        T b[D][D], t1[D][D], bn[D][D];
+        if (D == 2) {
+            b[0][0] = P.b[0][p]; b[1][1] = P.b[1][p]; b[0][1] = b[1][0] = P.b[2][p];
+        } else {
+            b[0][0] = P.b[0][p]; b[1][1] = P.b[1][p]; b[2][2] = P.b[2][p];
+            b[1][2] = b[2][1] = P.b[3][p]; b[0][2] = b[2][0] = P.b[4][p]; b[0][1] = b[1][0] = P.b[5][p];
+        }
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; j < D; j++) {
+                T a = dF[i][0] * b[0][j];
+#pragma unroll
+                for (int k = 1; k < D; k++) a += dF[i][k] * b[k][j];
+                t1[i][j] = a;
+            }
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = i; j < D; j++) {
+                T a = t1[i][0] * dF[j][0];
+#pragma unroll
+                for (int k = 1; k < D; k++) a += t1[i][k] * dF[j][k];
+                bn[i][j] = a;
+                bn[j][i] = a;
+            }
+        // ε = ½ log b (elast.jl:51-52), τ = Del voigt(ε, 2) (elast.jl:54)
+        T a[D][D], V[D][D];
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; j < D; j++) a[i][j] = bn[i][j];
+        sym_eig<D, T>(a, V);
+        T fl[D];
+#pragma unroll
+        for (int k = 0; k < D; k++) fl[k] = log(a[k][k]);
+        T e[D][D];
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; j < D; j++) {
+                T acc = T(0);
+#pragma unroll
+                for (int k = 0; k < D; k++) acc += (V[i][k] * fl[k]) * V[j][k];
+                e[i][j] = T(0.5) * acc;
+            }
+        T ev[NS];
+        if (D == 2) { ev[0] = e[0][0]; ev[1] = e[1][1]; ev[2] = T(2) * e[0][1]; }
+        else { ev[0] = e[0][0]; ev[1] = e[1][1]; ev[2] = e[2][2]; ev[3] = T(2) * e[1][2]; ev[4] = T(2) * e[0][2]; ev[5] = T(2) * e[0][1]; }
+#pragma unroll
+        for (int i = 0; i < NS; i++) {
+            T acc = T(0);
+#pragma unroll
+            for (int j = 0; j < NS; j++) acc += T(M.Del[i + NS * j]) * ev[j];
+            s[i] = acc;
+        }
+        if (D == 2) { P.b[0][p] = bn[0][0]; P.b[1][p] = bn[1][1]; P.b[2][p] = bn[0][1]; }
+        else { P.b[0][p] = bn[0][0]; P.b[1][p] = bn[1][1]; P.b[2][p] = bn[2][2]; P.b[3][p] = bn[1][2]; P.b[4][p] = bn[0][2]; P.b[5][p] = bn[0][1]; }
+    } else {
+        // infinitesimal_elast (elast.jl:69-80); the Jaumann term σω'+σ'ω is
+        // identically zero for symmetric σ
+        T ev[NS];
+        T e[D][D];
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; j < D; j++) e[i][j] = T(0.5) * (dF[i][j] + dF[j][i]) - (i == j ? T(1) : T(0));
+        if (D == 2) { ev[0] = e[0][0]; ev[1] = e[1][1]; ev[2] = T(2) * e[0][1]; }
+        else { ev[0] = e[0][0]; ev[1] = e[1][1]; ev[2] = e[2][2]; ev[3] = T(2) * e[1][2]; ev[4] = T(2) * e[0][2]; ev[5] = T(2) * e[0][1]; }
+#pragma unroll
+        for (int i = 0; i < NS; i++) {
+            T acc = T(0);
+#pragma unroll
+            for (int j = 0; j < NS; j++) acc += T(M.Del[i + NS * j]) * ev[j];
+            s[i] = P.s[i][p] + acc;
+        }
+    }
+    bool bnew = false;
+    if (c->with_plast && M.plast != 0) {
+        T e2 = P.epII[p], e1 = e2;
+        if (M.plast == 1) {
+            T dl;
+            dp_return<NS, T>(s, P.c0[p], P.cr[p], P.phi[p], T(M.Hp), T(M.Kc), T(M.Gc), e2, e1, dl);
+            P.dlam[p] = dl;
+            P.epII[p] = e1;
+            bnew = M.finite;
+        } else {
+            const int it = j2_return<NS, T>(s, P.c0[p], P.cr[p], T(M.Hp), M.Del, e2, e1);
+            if (it != 0) {
+                P.epII[p] = e1;
+                bnew = M.finite;
+                if (it < 0) atomicAdd(&c->j2cap, 1);
+            }
+        }
+    }
+    if (bnew) {
+        // ε = tensor(Del \ τ, ½); b = exp(2ε) (DP.jl:63-66)
+        T y[NS];
+        lu_solve<NS, T>(M.lu, s, y);
+        T a[D][D], V[D][D];
+        if (D == 2) { a[0][0] = y[0]; a[1][1] = y[1]; a[0][1] = a[1][0] = T(0.5) * y[2]; }
+        else {
+            a[0][0] = y[0]; a[1][1] = y[1]; a[2][2] = y[2];
+            a[1][2] = a[2][1] = T(0.5) * y[3]; a[0][2] = a[2][0] = T(0.5) * y[4]; a[0][1] = a[1][0] = T(0.5) * y[5];
+        }
+        sym_eig<D, T>(a, V);
+        T fe[D];
+#pragma unroll
+        for (int k = 0; k < D; k++) fe[k] = exp(T(2) * a[k][k]);
+        T bn[D][D];
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = i; j < D; j++) {
+                T acc = T(0);
+#pragma unroll
+                for (int k = 0; k < D; k++) acc += (V[i][k] * fe[k]) * V[j][k];
+                bn[i][j] = acc;
+            }
+        if (D == 2) { P.b[0][p] = bn[0][0]; P.b[1][p] = bn[1][1]; P.b[2][p] = bn[0][1]; }
+        else { P.b[0][p] = bn[0][0]; P.b[1][p] = bn[1][1]; P.b[2][p] = bn[2][2]; P.b[3][p] = bn[1][2]; P.b[4][p] = bn[0][2]; P.b[5][p] = bn[0][1]; }
+    }
+    bool bad = false;
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+        P.s[k][p] = s[k];
+        if (!isfinite(s[k])) bad = true;
+    }
+    if (bad) atomicOr(&c->err, ERR_NONFINITE);
+}
+
+// finite/infinitesimal_deform (deform.jl:1-45) + (F-bar off) elast/plast,
+// or (F-bar on) ΔJn scatter (volumetric.jl:1-11)
+template <int D, int B, typename T>
+__global__ void __launch_bounds__(256) k_deform(Ctrl *c, Geom g, PState<T> P, GState G, Mat M) {
+    const long p = blockIdx.x * long(blockDim.x) + threadIdx.x;
+    if (p >= P.n || !c->active) return;
+    const int par = c->parity;
+    T xp[D], lp[D];
+#pragma unroll
+    for (int i = 0; i < D; i++) { xp[i] = P.x[par][i][p]; lp[i] = B == 1 ? P.l0[i][p] : T(0); }
+    W<D, T> w;
+    weights<D, B, T>(g, xp, lp, w);
+    T gv[D][D];
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+        for (int j = 0; j < D; j++) gv[i][j] = T(0);
+    for_stencil<D, T>(g, w, [&](long no, T, const T *dN, const int *) {
+        T vn[D];
+#pragma unroll
+        for (int i = 0; i < D; i++) vn[i] = T(G.v[i][no]);
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; j < D; j++) gv[i][j] += dN[j] * vn[i];
+    });
+    const T dt = T(c->dt);
+    T dF[D][D], F[D][D], Fn[D][D];
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+            dF[i][j] = (i == j ? T(1) : T(0)) + dt * gv[i][j];
+            F[i][j] = P.F[i + D * j][p];
+        }
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+            T a = dF[i][0] * F[0][j];
+#pragma unroll
+            for (int k = 1; k < D; k++) a += dF[i][k] * F[k][j];
+            Fn[i][j] = a;
+        }
+    const T dJ = det<D, T>(dF);
+    const T J = det<D, T>(Fn);
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+            P.F[i + D * j][p] = Fn[i][j];
+            if (M.tpic) P.gv[i + D * j][p] = gv[i][j];
+        }
+    P.J[p] = J;
+    if (M.locking) {
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; j < D; j++) P.dF[i + D * j][p] = dF[i][j];
+        P.dJ[p] = dJ;
+        const T ms = mass_of(P, M, p, J);
+        const T q = ms * dJ;
+        for_stencil<D, T>(g, w, [&](long no, T N, const T *, const int *) { atomic_add(&G.dJ[no], double(N * q)); });
+    } else {
+        constitutive<D, T>(c, P, M, p, dF);
+    }
+}
+
+// ΔJs + ΔJp (volumetric.jl:12-35) fused, then elast/plast
+template <int D, int B, typename T>
+__global__ void __launch_bounds__(256) k_fbar(Ctrl *c, Geom g, PState<T> P, GState G, Mat M) {
+    const long p = blockIdx.x * long(blockDim.x) + threadIdx.x;
+    if (p >= P.n || !c->active) return;
+    const int par = c->parity;
+    T xp[D], lp[D];
+#pragma unroll
+    for (int i = 0; i < D; i++) { xp[i] = P.x[par][i][p]; lp[i] = B == 1 ? P.l0[i][p] : T(0); }
+    W<D, T> w;
+    weights<D, B, T>(g, xp, lp, w);
+    const T dJp = P.dJ[p];
+    T acc = T(0);
+    for_stencil<D, T>(g, w, [&](long no, T N, const T *, const int *) {
+        const double m = G.m[no];
+        const T q = m == 0.0 ? T(0) : T(G.dJ[no] / m);
+        acc += N * q / dJp;
+    });
+    const T f = pow(acc, T(1.0 / D));
+    T dF[D][D];
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+        for (int j = 0; j < D; j++) dF[i][j] = P.dF[i + D * j][p] * f;
+    constitutive<D, T>(c, P, M, p, dF);
+}
+
+}  // namespace ep

@@ -1,0 +1,243 @@
+"""ctypes binding of libelastoplasm_hip.so (include/elastoplasm.h).
+
+This is the same binding a Julia ``ccall`` shim makes: raw
+pointers into the column-major host arrays of ``Mesh`` / ``Point``.  There is
+no CPU fallback: if the library is missing or no GPU is visible, the calls
+fail loudly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+from .types import Mesh, Point
+
+LIBDIR = Path(__file__).resolve().parent.parent / "lib"
+LIBNAME = "libelastoplasm_hip.so"
+
+EP_OK = 0
+ERRORS = {-1: "EP_E_ARG", -2: "EP_E_CONFIG", -3: "EP_E_DOMAIN", -4: "EP_E_NONFINITE",
+          -5: "EP_E_HIP", -6: "EP_E_STATE"}
+
+_dp = C.POINTER(C.c_double)
+
+
+class EpConfig(C.Structure):
+    _fields_ = [("dim", C.c_int32), ("basis", C.c_int32), ("transfer", C.c_int32), ("musl", C.c_int32),
+                ("deform", C.c_int32), ("plast", C.c_int32), ("locking", C.c_int32), ("precision", C.c_int32),
+                ("nmp", C.c_int64), ("nel", C.c_int64 * 3), ("nno", C.c_int64 * 3),
+                ("h", C.c_double * 3), ("x0", C.c_double * 3), ("xB", (C.c_double * 2) * 3),
+                ("C_pf", C.c_double), ("damping", C.c_double), ("Kc", C.c_double), ("Gc", C.c_double),
+                ("Hp", C.c_double), ("c", C.c_double), ("Del", C.c_double * 36),
+                ("surrogate_tol", C.c_double), ("reserved", C.c_int32 * 8)]
+
+
+class EpGridView(C.Structure):
+    _fields_ = [("x", _dp), ("bcs", C.POINTER(C.c_uint8))]
+
+
+_PFIELDS = ["x", "v", "l0", "l", "F", "b", "gradv", "eps", "sig", "tau", "epII", "rho0", "rho", "Om0", "Om",
+            "n0", "n", "J", "dJ", "c0", "cr", "phi", "dlam", "vmax"]
+
+
+class EpParticlesView(C.Structure):
+    _fields_ = [(f, _dp) for f in _PFIELDS]
+
+
+class EpGridFields(C.Structure):
+    _fields_ = [("m", _dp), ("p", _dp), ("oobf", _dp), ("a", _dp), ("v", _dp)]
+
+
+class EpStepParams(C.Structure):
+    _fields_ = [("T", C.c_double), ("G", C.c_double), ("tg", C.c_double), ("ramp", C.c_int32),
+                ("with_plast", C.c_int32), ("dt_mode", C.c_int32), ("record_dt", C.c_int32),
+                ("dt_fixed", C.c_double)]
+
+
+class EpStepResult(C.Structure):
+    _fields_ = [("t", C.c_double), ("last_dt", C.c_double), ("steps", C.c_int64), ("it", C.c_int64)]
+
+
+class EpStats(C.Structure):
+    _fields_ = [("launches", C.c_int64), ("steps", C.c_int64), ("fallbacks", C.c_int64),
+                ("predictions", C.c_int64), ("j2_maxit", C.c_int64), ("max_residual", C.c_double)]
+
+
+EXPORTS = ["ep_version", "ep_last_error", "ep_create", "ep_destroy", "ep_upload_grid", "ep_upload_particles",
+           "ep_download_particles", "ep_download_grid", "ep_set_time", "ep_step", "ep_get_dts", "ep_profile",
+           "ep_kernel_times", "ep_get_stats"]
+
+_lib = None
+
+
+def lib_path() -> Path:
+    return Path(os.environ.get("EP_HIP_LIB", LIBDIR / LIBNAME))
+
+
+def load() -> C.CDLL:
+    """Load the HIP library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = lib_path()
+    if not p.exists():
+        raise RuntimeError(f"{p} is missing: run __graft_entry__.build() (make -C elastoplasm.jl_amd)")
+    L = C.CDLL(str(p))
+    L.ep_version.restype = C.c_int
+    L.ep_last_error.restype = C.c_char_p
+    L.ep_create.argtypes = [C.POINTER(EpConfig), C.POINTER(C.c_void_p)]
+    L.ep_destroy.argtypes = [C.c_void_p]
+    L.ep_destroy.restype = None
+    L.ep_upload_grid.argtypes = [C.c_void_p, C.POINTER(EpGridView)]
+    L.ep_upload_particles.argtypes = [C.c_void_p, C.POINTER(EpParticlesView)]
+    L.ep_download_particles.argtypes = [C.c_void_p, C.POINTER(EpParticlesView)]
+    L.ep_download_grid.argtypes = [C.c_void_p, C.POINTER(EpGridFields)]
+    L.ep_set_time.argtypes = [C.c_void_p, C.c_double]
+    L.ep_step.argtypes = [C.c_void_p, C.c_int32, C.POINTER(EpStepParams), C.POINTER(EpStepResult)]
+    L.ep_get_dts.argtypes = [C.c_void_p, _dp, C.c_int64, C.POINTER(C.c_int64)]
+    L.ep_profile.argtypes = [C.c_void_p, C.c_int32]
+    L.ep_kernel_times.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_int32, _dp,
+                                  C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
+    L.ep_get_stats.argtypes = [C.c_void_p, C.POINTER(EpStats)]
+    _lib = L
+    return L
+
+
+class EpError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def _check(r):
+    if r != EP_OK:
+        raise EpError(r, load().ep_last_error().decode())
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if not (a.flags["F_CONTIGUOUS"] and a.dtype == np.float64):
+        raise ValueError("arrays passed to the C ABI must be Float64 column-major")
+    return a.ctypes.data_as(_dp)
+
+
+def make_config(mesh: Mesh, mpts: Point, cmpr, instr) -> EpConfig:
+    ops = instr["ops"]
+    cfg = EpConfig()
+    d = mesh.dim
+    cfg.dim = d
+    cfg.basis, cfg.transfer, cfg.musl = ops["basis"], ops["transfer"], ops["musl"]
+    cfg.deform, cfg.plast, cfg.locking, cfg.precision = ops["deform"], ops["plast"], ops["locking"], ops["precision"]
+    cfg.nmp = mpts.nmp
+    for i in range(d):
+        cfg.nel[i] = int(mesh.nel[i])
+        cfg.nno[i] = int(mesh.nno[i])
+        cfg.h[i] = float(mesh.h[i])
+        cfg.x0[i] = float(mesh.x0[i])
+        cfg.xB[i][0] = float(mesh.xB[i, 0])
+        cfg.xB[i][1] = float(mesh.xB[i, 1])
+    cfg.C_pf, cfg.damping = ops["C_pf"], ops["damping"]
+    cfg.Kc, cfg.Gc, cfg.Hp, cfg.c = cmpr.Kc, cmpr.Gc, cmpr.Hp, cmpr.c
+    Del = np.asarray(cmpr.Del, dtype=np.float64).ravel(order="F")
+    for k, v in enumerate(Del):
+        cfg.Del[k] = v
+    cfg.surrogate_tol = float(instr.get("surrogate_tol", 1e-3))
+    return cfg
+
+
+def particles_view(mpts: Point) -> EpParticlesView:
+    s = mpts.s
+    v = EpParticlesView()
+    m = {"x": mpts.x, "v": s.v, "l0": mpts.l0, "l": mpts.l, "F": s.F, "b": s.b, "gradv": s.gradv, "eps": s.eps,
+         "sig": s.sig, "tau": s.tau, "epII": s.epII, "rho0": s.rho0, "rho": s.rho, "Om0": mpts.Om0, "Om": mpts.Om,
+         "n0": mpts.n0, "n": mpts.n, "J": mpts.J, "dJ": mpts.dJ, "c0": s.c0, "cr": s.cr, "phi": s.phi,
+         "dlam": s.dlam, "vmax": mpts.vmax}
+    for k, a in m.items():
+        setattr(v, k, _ptr(a))
+    return v
+
+
+class Device:
+    """One device context: a Mesh + Point pair resident in HBM."""
+
+    def __init__(self, mesh: Mesh, mpts: Point, cmpr, instr):
+        self.L = load()
+        self.mesh, self.mpts, self.cmpr, self.instr = mesh, mpts, cmpr, instr
+        self.cfg = make_config(mesh, mpts, cmpr, instr)
+        h = C.c_void_p()
+        _check(self.L.ep_create(C.byref(self.cfg), C.byref(h)))
+        self.h = h
+        self._bcs = np.asfortranarray(mesh.bcs.status.astype(np.uint8))
+        gv = EpGridView(_ptr(mesh.x), self._bcs.ctypes.data_as(C.POINTER(C.c_uint8)))
+        _check(self.L.ep_upload_grid(self.h, C.byref(gv)))
+        self.upload()
+
+    def upload(self, t: float | None = None):
+        _check(self.L.ep_upload_particles(self.h, C.byref(particles_view(self.mpts))))
+        if t is not None:
+            _check(self.L.ep_set_time(self.h, float(t)))
+
+    def set_time(self, t: float):
+        _check(self.L.ep_set_time(self.h, float(t)))
+
+    def download(self):
+        _check(self.L.ep_download_particles(self.h, C.byref(particles_view(self.mpts))))
+        return self.mpts
+
+    def download_grid(self):
+        me = self.mesh
+        f = EpGridFields(_ptr(me.m if me.m.flags["F_CONTIGUOUS"] else None), _ptr(me.p), _ptr(me.oobf),
+                         _ptr(me.a), _ptr(me.v))
+        _check(self.L.ep_download_grid(self.h, C.byref(f)))
+        return me
+
+    def step(self, nsteps: int, T: float, *, G: float = 9.81, tg: float = 0.0, ramp: bool = False,
+             plast: bool = True, dt: float | None = None, record_dt: bool = False):
+        p = EpStepParams(T=float(T), G=float(G), tg=float(tg), ramp=1 if ramp else 0,
+                         with_plast=1 if plast else 0, dt_mode=0 if dt is None else 1,
+                         record_dt=1 if record_dt else 0, dt_fixed=0.0 if dt is None else float(dt))
+        r = EpStepResult()
+        _check(self.L.ep_step(self.h, int(nsteps), C.byref(p), C.byref(r)))
+        return r
+
+    def dts(self, cap: int) -> np.ndarray:
+        out = np.zeros(cap)
+        n = C.c_int64()
+        _check(self.L.ep_get_dts(self.h, out.ctypes.data_as(_dp), cap, C.byref(n)))
+        return out[: n.value]
+
+    def profile(self, on: bool = True):
+        _check(self.L.ep_profile(self.h, 1 if on else 0))
+
+    def kernel_times(self) -> dict:
+        nmax, nl = 32, 32
+        names = C.create_string_buffer(nmax * nl)
+        ms = (C.c_double * nmax)()
+        la = (C.c_int64 * nmax)()
+        cnt = C.c_int32()
+        _check(self.L.ep_kernel_times(self.h, nmax, names, nl, ms, la, C.byref(cnt)))
+        out = {}
+        for k in range(cnt.value):
+            nm = names.raw[k * nl:(k + 1) * nl].split(b"\0", 1)[0].decode()
+            out[nm] = (ms[k], la[k])
+        return out
+
+    def stats(self) -> EpStats:
+        st = EpStats()
+        _check(self.L.ep_get_stats(self.h, C.byref(st)))
+        return st
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.ep_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,175 @@
+/*
+ * elastoplasm.h -- C ABI of libelastoplasm_hip.so, the MI355X (gfx950)
+ * implementation of ElastoPlasm.jl's explicit MPM step.
+ *
+ * The reference exposes its step as an operator table `instr.cairn` built by
+ * kwargser (src/home/api/kwargser.jl:49-59) and called once per step by the
+ * three wrappers shpfun / mapsto / elastoplast (src/home/program/workflow/
+ * worflow.jl:34-36, 80-82).  These entry points replace the bodies of those
+ * wrappers and of the time loop around them; Julia keeps its Point / Mesh
+ * structs (src/boot/needs/types.jl:38-167) and passes raw pointers into them.
+ *
+ * Conventions
+ *  - Every array argument points at a Julia column-major array: x[d,nmp] is
+ *    x[i + dim*p], F[d,d,nmp] is F[i + dim*(j + dim*p)], σᵢ[nstr,nmp] is
+ *    sig[c + nstr*p].  Values are Float64.  mesh.bcs.status is Matrix{Bool}
+ *    (one byte per entry).
+ *  - Every call returns EP_OK (0) or a negative EP_E_* code; ep_last_error()
+ *    gives the message of the last failed call on the calling thread.
+ *  - One context per host thread; a context owns one HIP stream and its
+ *    device buffers.  Host arrays are only read/written inside the call that
+ *    receives them; no pointer is retained.
+ */
+#ifndef ELASTOPLASM_H
+#define ELASTOPLASM_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EP_ABI_VERSION 1
+
+enum ep_status {
+    EP_OK = 0,
+    EP_E_ARG = -1,        /* null pointer, size out of range */
+    EP_E_CONFIG = -2,     /* unsupported basis/transfer/... (shpfun.jl:50, mapsto.jl:44, plast.jl:31) */
+    EP_E_DOMAIN = -3,     /* bsmpm "Invalid position" (bsmpm.jl:75) */
+    EP_E_NONFINITE = -4,  /* NaN/Inf in particle state after a step */
+    EP_E_HIP = -5,        /* HIP runtime failure */
+    EP_E_STATE = -6       /* call out of order (e.g. step before upload) */
+};
+
+enum ep_basis { EP_BSMPM = 0, EP_GIMPM = 1, EP_SMPM = 2 };         /* instr.basis.which */
+enum ep_transfer { EP_STD = 0, EP_TPIC = 1 };                      /* instr.fwrk.trsfr */
+enum ep_deform { EP_FINITE = 0, EP_INFINITESIMAL = 1 };            /* instr.fwrk.deform */
+enum ep_plast { EP_PLAST_NONE = 0, EP_DP = 1, EP_J2 = 2 };         /* instr.plast.constitutive */
+enum ep_precision { EP_FP64 = 0, EP_FP32 = 1, EP_FP32_SURROGATE = 2 };
+
+/* Setup-time description of one problem: instr + mesh scalars + cmpr.
+ * Replaces kwargser's cairn construction (kwargser.jl:49-59) and the
+ * scalar parts of setup_mesh / setup_cmpr (setup_mesh.jl:26-99,
+ * setup_cmpr.jl:120-143). */
+typedef struct ep_config {
+    int32_t dim;          /* 2 or 3 */
+    int32_t basis;        /* enum ep_basis */
+    int32_t transfer;     /* enum ep_transfer */
+    int32_t musl;         /* instr.fwrk.musl */
+    int32_t deform;       /* enum ep_deform */
+    int32_t plast;        /* enum ep_plast */
+    int32_t locking;      /* instr.fwrk.locking (F-bar) */
+    int32_t precision;    /* enum ep_precision */
+    int64_t nmp;          /* mpts.nmp */
+    int64_t nel[3];       /* mesh.nel[1:dim] */
+    int64_t nno[3];       /* mesh.nno[1:dim] */
+    double h[3];          /* mesh.h */
+    double x0[3];         /* mesh.x₀ */
+    double xB[3][2];      /* mesh.xB[axis, lo/hi] */
+    double C_pf;          /* instr.fwrk.C_pf */
+    double damping;       /* instr.fwrk.damping */
+    double Kc, Gc, Hp;    /* cmpr.Kc, cmpr.Gc, cmpr.Hp */
+    double c;             /* cmpr.c, elastic wave speed for get_dt */
+    double Del[36];       /* cmpr.Del, nstr x nstr column-major */
+    double surrogate_tol; /* EP_FP32_SURROGATE: yield-residual tolerance */
+    int32_t reserved[8];
+} ep_config;
+
+/* Pointers into mesh (types.jl:141-167). */
+typedef struct ep_grid_view {
+    const double *x;      /* mesh.x [dim, nno] */
+    const uint8_t *bcs;   /* mesh.bcs.status [dim, nno] */
+} ep_grid_view;
+
+/* Pointers into mpts / mpts.s (types.jl:38-105).  On upload the fields the
+ * configuration needs must be non-NULL; on download every non-NULL field is
+ * written.  nstr = 3 (2D) or 6 (3D). */
+typedef struct ep_particles_view {
+    double *x, *v;        /* [dim, nmp]  mpts.x, mpts.s.v */
+    double *l0;           /* [dim, nmp]  mpts.ℓ₀ (GIMP) */
+    double *l;            /* [dim, nmp]  mpts.ℓ (download) */
+    double *F, *b;        /* [dim,dim,nmp]  mpts.s.Fᵢⱼ, mpts.s.bᵢⱼ */
+    double *gradv;        /* [dim,dim,nmp]  mpts.s.∇vᵢⱼ (TPIC state) */
+    double *eps;          /* [dim,dim,nmp]  mpts.s.ϵᵢⱼ (download) */
+    double *sig, *tau;    /* [nstr, nmp]   mpts.s.σᵢ, mpts.s.τᵢ */
+    double *epII;         /* [2, nmp]      mpts.s.ϵpII */
+    double *rho0, *rho;   /* [nmp] mpts.s.ρ₀, mpts.s.ρ */
+    double *Om0, *Om;     /* [nmp] mpts.Ω₀, mpts.Ω */
+    double *n0, *n;       /* [nmp] mpts.n₀, mpts.n */
+    double *J, *dJ;       /* [nmp] mpts.J, mpts.ΔJ */
+    double *c0, *cr;      /* [nmp] mpts.s.c₀, mpts.s.cᵣ */
+    double *phi;          /* [nmp] mpts.s.ϕ */
+    double *dlam;         /* [nmp] mpts.s.Δλ */
+    double *vmax;         /* [dim] mpts.vmax */
+} ep_particles_view;
+
+/* Nodal fields for parity checks (download only), [dim,nno] / [nno]. */
+typedef struct ep_grid_fields {
+    double *m, *p, *oobf, *a, *v;
+} ep_grid_fields;
+
+/* One call of ep_step advances up to `nsteps` steps of elastodynamic! or
+ * elastoplastic! (worflow.jl:23-93) inside the window t < T.  Steps past
+ * T are skipped on the device (no host round trip per step). */
+typedef struct ep_step_params {
+    double T;             /* checkpoint time of the window (ΔT in get_dt) */
+    double G;             /* gravity magnitude, 9.81 in the reference */
+    double tg;            /* gravity ramp end time (time.tg) */
+    int32_t ramp;         /* 1: get_spacetime ramp (elastodynamic!), 0: get_g */
+    int32_t with_plast;   /* 1: elastoplast, 0: elasto */
+    int32_t dt_mode;      /* 0: get_dt CFL rule (physics.jl:21-26), 1: dt_fixed */
+    int32_t record_dt;    /* 1: keep each step's dt for ep_get_dts */
+    double dt_fixed;
+} ep_step_params;
+
+typedef struct ep_step_result {
+    double t;             /* time.t[1] after the call */
+    double last_dt;
+    int64_t steps;        /* steps executed by this call */
+    int64_t it;           /* total steps since upload */
+} ep_step_result;
+
+typedef struct ep_stats {
+    int64_t launches;
+    int64_t steps;
+    int64_t fallbacks;    /* surrogate predictions replaced by the FP64 return */
+    int64_t predictions;  /* surrogate predictions made */
+    int64_t j2_maxit;     /* J2 particles that hit the iteration cap */
+    double max_residual;  /* largest surrogate yield residual seen */
+} ep_stats;
+
+typedef struct ep_ctx ep_ctx;
+
+int ep_version(void);
+const char *ep_last_error(void);
+
+/* Replaces kwargser/init_* (kwargser.jl:49-59, shpfun.jl:14-60,
+ * mapsto.jl:13-54, update.jl:1-42, elast.jl:93-105, plast.jl:12-34). */
+int ep_create(const ep_config *cfg, ep_ctx **out);
+void ep_destroy(ep_ctx *ctx);
+
+/* Grid geometry and Dirichlet masks (setup_mesh.jl:115-188, get_bc.jl). */
+int ep_upload_grid(ep_ctx *ctx, const ep_grid_view *grid);
+/* Particle state (setup_mpts.jl:26-163 output or any later state). */
+int ep_upload_particles(ep_ctx *ctx, const ep_particles_view *mp);
+int ep_download_particles(ep_ctx *ctx, ep_particles_view *mp);
+int ep_download_grid(ep_ctx *ctx, ep_grid_fields *g);
+
+/* Time state: time.t[1] (and the step counter). */
+int ep_set_time(ep_ctx *ctx, double t);
+
+/* shpfun + mapsto + elasto/elastoplast for up to nsteps steps, with get_dt /
+ * get_spacetime on the device (worflow.jl:28-39, 74-85). Blocks until done. */
+int ep_step(ep_ctx *ctx, int32_t nsteps, const ep_step_params *par, ep_step_result *res);
+/* dt of each step of the last ep_step call with record_dt=1. */
+int ep_get_dts(ep_ctx *ctx, double *dts, int64_t cap, int64_t *n);
+
+/* Per-kernel HIP-event timing on the context's stream. */
+int ep_profile(ep_ctx *ctx, int32_t enable);
+int ep_kernel_times(ep_ctx *ctx, int32_t max_kernels, char *names, int32_t name_len,
+                    double *total_ms, int64_t *launches, int32_t *count);
+int ep_get_stats(ep_ctx *ctx, ep_stats *st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

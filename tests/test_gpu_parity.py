@@ -1,0 +1,163 @@
+"""Parity of the HIP path (through the C ABI) against the CPU oracle.
+
+Bar: plain FP64 matches the reference's CPU kernels to
+<= 1e-10 relative per step, and <= 1e-6 relative in displacement and stress
+after 100 steps.  Errors are norm-wise over all particles,
+||dev - ref|| / ||ref||.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from cases import clone, compare, field, relerr, slump
+from elastoplasm import hip
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+STEP_TOL = 1e-10
+RUN_TOL = 1e-6
+
+# C1 materials (2D slump of the BASELINE config list)
+C1 = dict(rho0=2650.0, E=1.0e6, nu=0.3, c0=20.0e3, cr=20.0e3, phi0=math.radians(30.0),
+          phir=math.radians(30.0), Hp=0.0)
+
+
+def _device_run(ic, cfg, nsteps, T, *, t0=0.0, plast=True, ramp=False, tg=0.0, record=False):
+    dev = hip.Device(ic.mesh, ic.mpts, ic.cmpr, cfg.instr)
+    try:
+        dev.set_time(t0)
+        r = dev.step(nsteps, T, plast=plast, ramp=ramp, tg=tg, record_dt=record)
+        dts = dev.dts(nsteps) if record else None
+        dev.download()
+    finally:
+        dev.close()
+    return r, dts
+
+
+def _warm_state(ic, cfg, nwarm, T=100.0):
+    """Advance with the oracle to a state with motion and plasticity."""
+    o = Oracle(ic.mesh, ic.mpts, ic.cmpr, cfg.instr)
+    o.run(nwarm, T, plast=True)
+    return o.t
+
+
+def _one_step(ic, cfg, nwarm, fields, tol=STEP_TOL, T=100.0):
+    t0 = _warm_state(ic, cfg, nwarm, T) if nwarm else 0.0
+    ref, dev = clone(ic), clone(ic)
+    o = Oracle(ref.mesh, ref.mpts, ref.cmpr, cfg.instr)
+    o.t = t0
+    o.run(1, T, plast=True)
+    r, _ = _device_run(dev, cfg, 1, T, t0=t0)
+    assert r.steps == 1
+    err = compare(dev.mpts, ref.mpts, fields)
+    print("one-step", err)
+    for f, e in err.items():
+        assert e <= tol, (f, e)
+    return err
+
+
+CASES_2D = [
+    dict(deform="finite", trsfr="std", musl=True, locking=True),
+    dict(deform="finite", trsfr="std", musl=True, locking=False),
+    dict(deform="finite", trsfr="tpic", musl=True, locking=True),
+    dict(deform="finite", trsfr="tpic", musl=False, locking=True),
+    dict(deform="infinitesimal", trsfr="std", musl=True, locking=True),
+    dict(deform="infinitesimal", trsfr="std", musl=True, locking=False),
+    dict(deform="infinitesimal", trsfr="tpic", musl=False, locking=False),
+]
+
+
+@pytest.mark.parametrize("case", CASES_2D, ids=lambda c: "-".join(f"{k}={v}" for k, v in c.items()))
+def test_one_step_2d_bsmpm(case):
+    ic, cfg = slump(2, **case)
+    fields = ["x", "v", "F", "J", "epII", "dlam"] + (["b", "tau"] if case["deform"] == "finite" else ["sig"])
+    if case["trsfr"] == "tpic":
+        fields.append("gradv")
+    _one_step(ic, cfg, 40, fields)
+
+
+@pytest.mark.parametrize("basis", ["gimpm", "smpm"])
+def test_one_step_2d_other_bases(basis):
+    ic, cfg = slump(2, basis=basis)
+    _one_step(ic, cfg, 30, ["x", "v", "F", "J", "b", "tau", "epII"])
+
+
+def test_one_step_2d_j2():
+    ic, cfg = slump(2, plast="J2", cmpr=dict(c0=8.0e3, cr=4.0e3, Hp=1.0e4))
+    _one_step(ic, cfg, 40, ["x", "v", "F", "J", "b", "tau", "epII"])
+
+
+def test_one_step_3d():
+    ic, cfg = slump(3)
+    _one_step(ic, cfg, 10, ["x", "v", "F", "J", "b", "tau", "epII", "dlam"])
+
+
+def test_one_step_3d_gimp_infinitesimal_j2():
+    ic, cfg = slump(3, basis="gimpm", deform="infinitesimal", plast="J2",
+                    cmpr=dict(c0=8.0e3, cr=4.0e3, Hp=1.0e4))
+    _one_step(ic, cfg, 10, ["x", "v", "F", "J", "sig", "epII"])
+
+
+def test_c1_100_steps():
+    """C1: 2D slump, DP c=20 kPa, phi=30, rho 2650, bsmpm, mUSL-FLIP, finite,
+    CFL 0.5, 100 steps from t=0."""
+    ic, cfg = slump(2, C_pf=1.0, cmpr=C1)
+    x0 = ic.mpts.x.copy(order="F")
+    ref, dev = clone(ic), clone(ic)
+    o = Oracle(ref.mesh, ref.mpts, ref.cmpr, cfg.instr)
+    dts_ref = o.run(100, 100.0, plast=True)
+    r, dts = _device_run(dev, cfg, 100, 100.0, record=True)
+    assert r.steps == 100
+    assert relerr(dts, dts_ref) <= 1e-9
+    assert abs(r.t - o.t) <= 1e-9 * o.t
+    du = relerr(dev.mpts.x - x0, ref.mpts.x - x0)
+    dtau = relerr(dev.mpts.s.tau, ref.mpts.s.tau)
+    print("C1 100 steps: displacement", du, "stress", dtau, "plastic particles",
+          int((ref.mpts.s.epII[0] > 0).sum()))
+    assert (ref.mpts.s.epII[0] > 0).sum() > 0
+    assert du <= RUN_TOL and dtau <= RUN_TOL
+
+
+def test_3d_20_steps():
+    ic, cfg = slump(3, C_pf=1.0, cmpr=C1)
+    x0 = ic.mpts.x.copy(order="F")
+    ref, dev = clone(ic), clone(ic)
+    o = Oracle(ref.mesh, ref.mpts, ref.cmpr, cfg.instr)
+    o.run(20, 100.0, plast=True)
+    r, _ = _device_run(dev, cfg, 20, 100.0)
+    du = relerr(dev.mpts.x - x0, ref.mpts.x - x0)
+    dtau = relerr(dev.mpts.s.tau, ref.mpts.s.tau)
+    print("3D 20 steps: displacement", du, "stress", dtau)
+    assert du <= RUN_TOL and dtau <= RUN_TOL
+
+
+def test_window_stops_at_T():
+    """Device-side `while T > t` guard: steps beyond the window are skipped
+    and the last dt is clipped to T - t (physics.jl:24)."""
+    ic, cfg = slump(2)
+    ref, dev = clone(ic), clone(ic)
+    o = Oracle(ref.mesh, ref.mpts, ref.cmpr, cfg.instr)
+    T = 0.5
+    dts_ref = o.run(1000, T)
+    r, dts = _device_run(dev, cfg, 64, T, record=True)
+    assert r.steps == len(dts_ref) < 64
+    assert r.t == pytest.approx(T, rel=0, abs=1e-12)
+    assert relerr(dts, dts_ref) <= 1e-9
+    assert relerr(dev.mpts.x, ref.mpts.x) <= 1e-10
+
+
+def test_elastodynamic_ramp_parity():
+    """elastodynamic! with the gravity ramp of get_spacetime."""
+    ic, cfg = slump(2)
+    ref, dev = clone(ic), clone(ic)
+    o = Oracle(ref.mesh, ref.mpts, ref.cmpr, cfg.instr)
+    o.run(50, 10.0, ramp=True, tg=10.0, plast=False)
+    r, _ = _device_run(dev, cfg, 50, 10.0, ramp=True, tg=10.0, plast=False)
+    assert r.steps == 50
+    err = compare(dev.mpts, ref.mpts, ["x", "v", "F", "b", "tau"])
+    print("elastodynamic", err)
+    assert max(err.values()) <= RUN_TOL
