@@ -7,8 +7,6 @@
 #include "../../include/elastoplasm.h"
 #include "ep_kernels.h"
 
-#include <hip/hip_runtime.h>
-
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -74,14 +72,14 @@ struct ep_ctx {
     // grid
     Geom geom{};
     GState gs{};
-    double *grid_block = nullptr;
+    std::vector<void *> grid_allocs;   // one allocation per nodal field
     double *axis_block = nullptr;
     int8_t *type_block = nullptr;
     uint8_t *bc_dev = nullptr;
     bool grid_ready = false;
     // particles
     PState<double> ps{};
-    double *part_block = nullptr;
+    std::vector<void *> part_allocs;   // one allocation per particle field
     long nmp = 0;
     std::vector<long> perm;          // device slot -> host particle index
     std::vector<double> n0_h, rho0_h, Om0_h;
@@ -166,11 +164,11 @@ extern "C" int ep_create(const ep_config *cfg, ep_ctx **out) {
 extern "C" void ep_destroy(ep_ctx *c) {
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
-    hipFree(c->grid_block);
+    for (void *a : c->grid_allocs) hipFree(a);
     hipFree(c->axis_block);
     hipFree(c->type_block);
     hipFree(c->bc_dev);
-    hipFree(c->part_block);
+    for (void *a : c->part_allocs) hipFree(a);
     hipFree(c->ctrl);
     hipFree(c->dt_log);
     for (auto e : c->ev_pool) hipEventDestroy(e);
@@ -230,8 +228,9 @@ extern "C" int ep_upload_grid(ep_ctx *c, const ep_grid_view *gv) {
     hipFree(c->axis_block);
     hipFree(c->type_block);
     hipFree(c->bc_dev);
-    hipFree(c->grid_block);
-    c->axis_block = nullptr; c->type_block = nullptr; c->bc_dev = nullptr; c->grid_block = nullptr;
+    for (void *a : c->grid_allocs) hipFree(a);
+    c->grid_allocs.clear();
+    c->axis_block = nullptr; c->type_block = nullptr; c->bc_dev = nullptr;
     HIPCHK(hipMalloc(&c->axis_block, sizeof(double) * ax.size()));
     HIPCHK(hipMalloc(&c->type_block, ty.size()));
     HIPCHK(hipMalloc(&c->bc_dev, nno));
@@ -242,19 +241,25 @@ extern "C" int ep_upload_grid(ep_ctx *c, const ep_grid_view *gv) {
         g.X[i] = i < D ? c->axis_block + off[i] : c->axis_block;
         g.T[i] = i < D ? c->type_block + off[i] : c->type_block;
     }
-    // m, p[3], f[3], a[3], v[3], pa[3], dJ  -> 15 fields
-    const int nf = 15;
-    HIPCHK(hipMalloc(&c->grid_block, sizeof(double) * nf * nno));
-    HIPCHK(hipMemset(c->grid_block, 0, sizeof(double) * nf * nno));
+    // m, p[D], f[D], a[D], v[D], pa[D], dJ: one allocation each
     GState &G = c->gs;
-    double *b = c->grid_block;
-    G.m = b; b += nno;
-    for (int d = 0; d < 3; d++) { G.p[d] = b; b += nno; }
-    for (int d = 0; d < 3; d++) { G.f[d] = b; b += nno; }
-    for (int d = 0; d < 3; d++) { G.a[d] = b; b += nno; }
-    for (int d = 0; d < 3; d++) { G.v[d] = b; b += nno; }
-    for (int d = 0; d < 3; d++) { G.pa[d] = b; b += nno; }
-    G.dJ = b;
+    G = GState{};
+    auto galloc = [&](double **dst) -> int {
+        void *a = nullptr;
+        HIPCHK(hipMalloc(&a, sizeof(double) * nno));
+        HIPCHK(hipMemset(a, 0, sizeof(double) * nno));
+        c->grid_allocs.push_back(a);
+        *dst = static_cast<double *>(a);
+        return EP_OK;
+    };
+    int r;
+    if ((r = galloc(&G.m)) != EP_OK) return r;
+    for (int d = 0; d < D; d++) {
+        if ((r = galloc(&G.p[d])) != EP_OK || (r = galloc(&G.f[d])) != EP_OK || (r = galloc(&G.a[d])) != EP_OK ||
+            (r = galloc(&G.v[d])) != EP_OK || (r = galloc(&G.pa[d])) != EP_OK)
+            return r;
+    }
+    if ((r = galloc(&G.dJ)) != EP_OK) return r;
     G.bc = c->bc_dev;
     G.nno = nno;
     c->grid_ready = true;
@@ -293,23 +298,19 @@ extern "C" int ep_upload_particles(ep_ctx *c, const ep_particles_view *v) {
     c->perm.resize(n);
     std::iota(c->perm.begin(), c->perm.end(), 0L);
     std::stable_sort(c->perm.begin(), c->perm.end(), [&](long a, long b) { return key[a] < key[b]; });
-    // field table
-    struct F { double **dst; int kind; int comp; };
-    // allocate
-    int nf = 6 + 3 + DD + (D == 2 ? 3 : 6) + NS + 8;
-    if (c->cfg.basis == EP_GIMPM) nf += D;
-    if (M.tpic) nf += DD;
-    if (M.locking) nf += DD + 1;
-    if (c->nmp != n || !c->part_block) {
-        hipFree(c->part_block);
-        c->part_block = nullptr;
-        HIPCHK(hipMalloc(&c->part_block, sizeof(double) * size_t(nf) * n));
-        c->nmp = n;
-    }
+    // one device allocation per particle field
+    for (void *a : c->part_allocs) hipFree(a);
+    c->part_allocs.clear();
+    c->nmp = n;
     PState<double> &P = c->ps;
     P = PState<double>{};
-    double *b = c->part_block;
-    auto take = [&]() { double *r = b; b += n; return r; };
+    bool oom = false;
+    auto take = [&]() -> double * {
+        void *a = nullptr;
+        if (hipMalloc(&a, sizeof(double) * n) != hipSuccess) { oom = true; return nullptr; }
+        c->part_allocs.push_back(a);
+        return static_cast<double *>(a);
+    };
     for (int k = 0; k < 2; k++) for (int i = 0; i < D; i++) P.x[k][i] = take();
     for (int i = 0; i < D; i++) P.v[i] = take();
     for (int i = 0; i < DD; i++) P.F[i] = take();
@@ -320,6 +321,7 @@ extern "C" int ep_upload_particles(ep_ctx *c, const ep_particles_view *v) {
     if (c->cfg.basis == EP_GIMPM) for (int i = 0; i < D; i++) P.l0[i] = take();
     if (M.tpic) for (int i = 0; i < DD; i++) P.gv[i] = take();
     if (M.locking) { for (int i = 0; i < DD; i++) P.dF[i] = take(); P.dJ = take(); }
+    if (oom) return fail(EP_E_HIP, "hipMalloc of particle fields failed");
     P.n = n;
     // host staging in device order
     std::vector<double> st(n);
@@ -506,7 +508,7 @@ static void timed(ep_ctx *c, int kid, L &&launch) {
 
 template <int D, int B>
 static void launch_step(ep_ctx *c) {
-    const int TB = 256;
+    const int TB = EP_BLOCK;
     const long n = c->nmp, nno = c->gs.nno;
     const dim3 gp((n + TB - 1) / TB), gg((nno + TB - 1) / TB);
     hipStream_t s = c->stream;
@@ -515,15 +517,15 @@ static void launch_step(ep_ctx *c) {
     const GState G = c->gs;
     const PState<double> P = c->ps;
     const Mat M = c->mat;
-    timed(c, K_BEGIN, [&] { k_begin<D><<<1, 64, 0, s>>>(ct, g, M); });
-    timed(c, K_ZERO, [&] { k_zero_grid<D><<<gg, TB, 0, s>>>(ct, G); });
-    timed(c, K_P2G, [&] { k_p2g<D, B, double><<<gp, TB, 0, s>>>(ct, g, P, G, M); });
-    timed(c, K_EULER, [&] { k_euler<D><<<gg, TB, 0, s>>>(ct, G, M); });
-    timed(c, K_G2P, [&] { k_g2p<D, B, double><<<gp, TB, 0, s>>>(ct, g, P, G, M); });
-    if (M.musl) timed(c, K_AUGV, [&] { k_augv<D><<<gg, TB, 0, s>>>(ct, G); });
-    timed(c, K_DEFORM, [&] { k_deform<D, B, double><<<gp, TB, 0, s>>>(ct, g, P, G, M); });
-    if (M.locking) timed(c, K_FBAR, [&] { k_fbar<D, B, double><<<gp, TB, 0, s>>>(ct, g, P, G, M); });
-    timed(c, K_END, [&] { k_end<<<1, 64, 0, s>>>(ct); });
+    timed(c, K_BEGIN, [&] { EP_LAUNCH(k_begin<D>, 1, 64, s, ct, g, M); });
+    timed(c, K_ZERO, [&] { EP_LAUNCH(k_zero_grid<D>, gg, TB, s, ct, G); });
+    timed(c, K_P2G, [&] { EP_LAUNCH((k_p2g<D, B, double>), gp, TB, s, ct, g, P, G, M); });
+    timed(c, K_EULER, [&] { EP_LAUNCH(k_euler<D>, gg, TB, s, ct, G, M); });
+    timed(c, K_G2P, [&] { EP_LAUNCH((k_g2p<D, B, double>), gp, TB, s, ct, g, P, G, M); });
+    if (M.musl) timed(c, K_AUGV, [&] { EP_LAUNCH(k_augv<D>, gg, TB, s, ct, G); });
+    timed(c, K_DEFORM, [&] { EP_LAUNCH((k_deform<D, B, double>), gp, TB, s, ct, g, P, G, M); });
+    if (M.locking) timed(c, K_FBAR, [&] { EP_LAUNCH((k_fbar<D, B, double>), gp, TB, s, ct, g, P, G, M); });
+    timed(c, K_END, [&] { EP_LAUNCH(k_end, 1, 64, s, ct); });
 }
 
 static void dispatch_step(ep_ctx *c) {

@@ -183,6 +183,108 @@ __device__ __forceinline__ T mass_of(const PState<T> &P, const Mat &M, long p, T
     return M.finite ? (P.rho[p] / J) * Om : P.rho[p] * Om;
 }
 
+
+// -------------------------------------------------------------------------
+// Block scatter accumulator.  A block of 256 cell-sorted particles touches a
+// small box of nodes; the block reduces the box of its stencils, accumulates
+// its contributions in LDS (ds_add_f64) and flushes the box with one global
+// atomic per node value.  When the box does not fit the LDS budget (a block
+// spanning distant cells) contributions go straight to global atomics.
+// -------------------------------------------------------------------------
+constexpr int ACC_LDS = 4096;   // doubles of LDS per block (32 KiB)
+
+template <int D, int NV>
+struct BlockAcc {
+    int lo[D], ext[D];
+    int nloc;
+    bool lds;
+    double *buf;
+
+    __device__ __forceinline__ void begin(double *smem, int *sint, bool live, const int *base) {
+        buf = smem;
+        if (threadIdx.x < D) { sint[threadIdx.x] = 0x7fffffff; sint[D + threadIdx.x] = -0x7fffffff; }
+        __syncthreads();
+        if (live) {
+#pragma unroll
+            for (int i = 0; i < D; i++) { atomicMin(&sint[i], base[i]); atomicMax(&sint[D + i], base[i]); }
+        }
+        __syncthreads();
+        nloc = 1;
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+            lo[i] = sint[i];
+            ext[i] = sint[D + i] - sint[i] + 4;
+            nloc *= ext[i];
+        }
+        if (sint[0] == 0x7fffffff) nloc = 0;
+        lds = nloc > 0 && nloc * NV <= ACC_LDS;
+        if (lds) {
+            for (int k = threadIdx.x; k < nloc * NV; k += blockDim.x) buf[k] = 0.0;
+        }
+        __syncthreads();
+    }
+    // local index: last axis (z) fastest, then x, then y, as the grid
+    __device__ __forceinline__ int local(const int *ix) const {
+        if (D == 2) return (ix[1] - lo[1]) + ext[1] * (ix[0] - lo[0]);
+        return (ix[2] - lo[2]) + ext[2] * ((ix[0] - lo[0]) + ext[0] * (ix[1] - lo[1]));
+    }
+    __device__ __forceinline__ void add(double *const *dst, int v, long no, int li, double val) {
+        if (lds) unsafeAtomicAdd(&buf[v * nloc + li], val);
+        else unsafeAtomicAdd(&dst[v][no], val);
+    }
+    __device__ __forceinline__ void flush(double *const *dst, const Geom &g) {
+        if (!lds) return;
+        __syncthreads();
+        for (int k = threadIdx.x; k < nloc; k += blockDim.x) {
+            int ix[3];
+            long no;
+            if (D == 2) {
+                ix[1] = lo[1] + k % ext[1];
+                ix[0] = lo[0] + k / ext[1];
+                if (ix[0] < 0 || ix[1] < 0 || ix[0] >= g.nno[0] || ix[1] >= g.nno[1]) continue;
+                no = long(ix[1]) + long(g.nno[1]) * ix[0];
+            } else {
+                ix[2] = lo[2] + k % ext[2];
+                const int r = k / ext[2];
+                ix[0] = lo[0] + r % ext[0];
+                ix[1] = lo[1] + r / ext[0];
+                if (ix[0] < 0 || ix[1] < 0 || ix[2] < 0 || ix[0] >= g.nno[0] || ix[1] >= g.nno[1] ||
+                    ix[2] >= g.nno[2])
+                    continue;
+                no = long(ix[2]) + long(g.nno[2]) * (long(ix[0]) + long(g.nno[0]) * ix[1]);
+            }
+#pragma unroll
+            for (int v = 0; v < NV; v++) {
+                const double val = buf[v * nloc + k];
+                if (val != 0.0) unsafeAtomicAdd(&dst[v][no], val);
+            }
+        }
+    }
+};
+
+// Block max of non-negative doubles into a global bit-pattern max.
+__device__ __forceinline__ void block_max_pos(unsigned long long *dst, double v, unsigned *sred) {
+    unsigned long long b = (unsigned long long)__double_as_longlong(v);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const unsigned long long o = __shfl_xor(b, off);
+        b = o > b ? o : b;
+    }
+    (void)sred;
+    if ((threadIdx.x & 63) == 0) atomicMax(dst, b);
+}
+
+template <int D, typename T>
+__device__ __forceinline__ void stencil_base(const Geom &g, const T *xp, int *base) {
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+        double e = exact_fld(double(xp[i]) - g.x0[i], g.h[i]);
+        const double hi = double(g.nel[i] - 1);
+        e = e < 0.0 ? 0.0 : (e > hi ? hi : e);
+        base[i] = int(e) - 1;
+    }
+}
+
 // -------------------------------------------------------------------------
 // step kernels
 // -------------------------------------------------------------------------
@@ -240,65 +342,79 @@ __global__ void k_zero_grid(const Ctrl *c, GState G) {
 template <int D, int B, typename T>
 __global__ void __launch_bounds__(256) k_p2g(const Ctrl *c, Geom g, PState<T> P, GState G, Mat M) {
     constexpr int NS = D == 2 ? 3 : 6;
+    constexpr int NV = 1 + 2 * D;
+    __shared__ double smem[ACC_LDS];
+    __shared__ int sint[8];
+    if (!c->active) return;
     const long p = blockIdx.x * long(blockDim.x) + threadIdx.x;
-    if (p >= P.n || !c->active) return;
+    const bool live = p < P.n;
+    const long q = live ? p : 0;
     const int par = c->parity;
     T xp[D], lp[D];
 #pragma unroll
-    for (int i = 0; i < D; i++) { xp[i] = P.x[par][i][p]; lp[i] = B == 1 ? P.l0[i][p] : T(0); }
+    for (int i = 0; i < D; i++) { xp[i] = P.x[par][i][q]; lp[i] = B == 1 ? P.l0[i][q] : T(0); }
     W<D, T> w;
-    if (!weights<D, B, T>(g, xp, lp, w)) atomicOr(const_cast<unsigned *>(&c->err), ERR_DOMAIN);
-    const T J = P.J[p];
-    const T Om = J * P.Om0[p];
-    const T ms = mass_of(P, M, p, J);
+    if (!weights<D, B, T>(g, xp, lp, w) && live) atomicOr(const_cast<unsigned *>(&c->err), ERR_DOMAIN);
+    if (!live) w.ok = 0u;
+    BlockAcc<D, NV> acc;
+    acc.begin(smem, sint, live, w.base);
+    double *dst[NV];
+    dst[0] = G.m;
+#pragma unroll
+    for (int d = 0; d < D; d++) { dst[1 + d] = G.p[d]; dst[1 + D + d] = G.f[d]; }
+    const T J = P.J[q];
+    const T Om = J * P.Om0[q];
+    const T ms = mass_of(P, M, q, J);
     T s[NS];
 #pragma unroll
-    for (int k = 0; k < NS; k++) s[k] = M.finite ? P.s[k][p] / J : P.s[k][p];
+    for (int k = 0; k < NS; k++) s[k] = M.finite ? P.s[k][q] / J : P.s[k][q];
     T v[D], pm[D], gv[D][D];
 #pragma unroll
-    for (int i = 0; i < D; i++) { v[i] = P.v[i][p]; pm[i] = ms * v[i]; }
+    for (int i = 0; i < D; i++) { v[i] = P.v[i][q]; pm[i] = ms * v[i]; }
     if (M.tpic) {
 #pragma unroll
         for (int i = 0; i < D; i++)
 #pragma unroll
-            for (int j = 0; j < D; j++) gv[i][j] = P.gv[i + D * j][p];
+            for (int j = 0; j < D; j++) gv[i][j] = P.gv[i + D * j][q];
     }
     const T gz = T(c->g[D - 1]);
     const bool tpic = M.tpic;
     for_stencil<D, T>(g, w, [&](long no, T N, const T *dN, const int *ix) {
-        atomic_add(&G.m[no], double(N * ms));
+        const int li = acc.local(ix);
+        acc.add(dst, 0, no, li, double(N * ms));
         if (D == 2) {
             if (tpic) {
                 // Δnp = x_n - x_p (misc.jl:1-14)
                 const T dx = T(g.X[0][ix[0]]) - xp[0];
                 const T dy = T(g.X[1][ix[1]]) - xp[1];
-                atomic_add(&G.p[0][no], double(N * ms * (v[0] + gv[0][0] * dx + gv[0][1] * dy)));
-                atomic_add(&G.p[1][no], double(N * ms * (v[1] + gv[1][0] * dx + gv[1][1] * dy)));
+                acc.add(dst, 1, no, li, double(N * ms * (v[0] + gv[0][0] * dx + gv[0][1] * dy)));
+                acc.add(dst, 2, no, li, double(N * ms * (v[1] + gv[1][0] * dx + gv[1][1] * dy)));
             } else {
-                atomic_add(&G.p[0][no], double(N * pm[0]));
-                atomic_add(&G.p[1][no], double(N * pm[1]));
+                acc.add(dst, 1, no, li, double(N * pm[0]));
+                acc.add(dst, 2, no, li, double(N * pm[1]));
             }
-            atomic_add(&G.f[0][no], double(-(Om * (dN[0] * s[0] + dN[1] * s[2]))));
-            atomic_add(&G.f[1][no], double(-(Om * (dN[0] * s[2] + dN[1] * s[1]) - N * (ms * gz))));
+            acc.add(dst, 3, no, li, double(-(Om * (dN[0] * s[0] + dN[1] * s[2]))));
+            acc.add(dst, 4, no, li, double(-(Om * (dN[0] * s[2] + dN[1] * s[1]) - N * (ms * gz))));
         } else {
             if (tpic) {
                 const T dx = T(g.X[0][ix[0]]) - xp[0];
                 const T dy = T(g.X[1][ix[1]]) - xp[1];
                 const T dz = T(g.X[2][ix[2]]) - xp[2];
-                atomic_add(&G.p[0][no], double(N * ms * (v[0] + gv[0][0] * dx + gv[0][1] * dy + gv[0][2] * dz)));
-                atomic_add(&G.p[1][no], double(N * ms * (v[1] + gv[1][0] * dx + gv[1][1] * dy + gv[1][2] * dz)));
-                atomic_add(&G.p[2][no], double(N * ms * (v[2] + gv[2][0] * dx + gv[2][1] * dy + gv[2][2] * dz)));
+                acc.add(dst, 1, no, li, double(N * ms * (v[0] + gv[0][0] * dx + gv[0][1] * dy + gv[0][2] * dz)));
+                acc.add(dst, 2, no, li, double(N * ms * (v[1] + gv[1][0] * dx + gv[1][1] * dy + gv[1][2] * dz)));
+                acc.add(dst, 3, no, li, double(N * ms * (v[2] + gv[2][0] * dx + gv[2][1] * dy + gv[2][2] * dz)));
             } else {
-                atomic_add(&G.p[0][no], double(N * pm[0]));
-                atomic_add(&G.p[1][no], double(N * pm[1]));
-                atomic_add(&G.p[2][no], double(N * pm[2]));
+                acc.add(dst, 1, no, li, double(N * pm[0]));
+                acc.add(dst, 2, no, li, double(N * pm[1]));
+                acc.add(dst, 3, no, li, double(N * pm[2]));
             }
             // Voigt [xx,yy,zz,yz,xz,xy]: σyx = s[5], σzy = s[3], σzx = s[4]
-            atomic_add(&G.f[0][no], double(-(Om * (dN[0] * s[0] + dN[1] * s[5] + dN[2] * s[4]))));
-            atomic_add(&G.f[1][no], double(-(Om * (dN[0] * s[5] + dN[1] * s[1] + dN[2] * s[3]))));
-            atomic_add(&G.f[2][no], double(-(Om * (dN[0] * s[4] + dN[1] * s[3] + dN[2] * s[2]) - N * (ms * gz))));
+            acc.add(dst, 4, no, li, double(-(Om * (dN[0] * s[0] + dN[1] * s[5] + dN[2] * s[4]))));
+            acc.add(dst, 5, no, li, double(-(Om * (dN[0] * s[5] + dN[1] * s[1] + dN[2] * s[3]))));
+            acc.add(dst, 6, no, li, double(-(Om * (dN[0] * s[4] + dN[1] * s[3] + dN[2] * s[2]) - N * (ms * gz))));
         }
     });
+    acc.flush(dst, g);
 }
 
 // euler (solve.jl:14-40); a and v of massless nodes are the zeros that
@@ -342,14 +458,19 @@ __global__ void k_euler(const Ctrl *c, GState G, Mat M) {
 // picflip_n2p + augm_momentum (n2p.jl:14-40, augm.jl:13-25)
 template <int D, int B, typename T>
 __global__ void __launch_bounds__(256) k_g2p(Ctrl *c, Geom g, PState<T> P, GState G, Mat M) {
+    __shared__ double smem[ACC_LDS];
+    __shared__ int sint[8];
+    if (!c->active) return;
     const long p = blockIdx.x * long(blockDim.x) + threadIdx.x;
-    if (p >= P.n || !c->active) return;
+    const bool live = p < P.n;
+    const long q = live ? p : 0;
     const int par = c->parity;
     T xp[D], lp[D];
 #pragma unroll
-    for (int i = 0; i < D; i++) { xp[i] = P.x[par][i][p]; lp[i] = B == 1 ? P.l0[i][p] : T(0); }
+    for (int i = 0; i < D; i++) { xp[i] = P.x[par][i][q]; lp[i] = B == 1 ? P.l0[i][q] : T(0); }
     W<D, T> w;
     weights<D, B, T>(g, xp, lp, w);
+    if (!live) w.ok = 0u;
     T vpic[D], aflip[D];
 #pragma unroll
     for (int d = 0; d < D; d++) { vpic[d] = T(0); aflip[d] = T(0); }
@@ -365,20 +486,29 @@ __global__ void __launch_bounds__(256) k_g2p(Ctrl *c, Geom g, PState<T> P, GStat
     bool bad = false;
 #pragma unroll
     for (int d = 0; d < D; d++) {
-        vn[d] = C * (P.v[d][p] + dt * aflip[d]) + (T(1) - C) * vpic[d];
+        vn[d] = C * (P.v[d][q] + dt * aflip[d]) + (T(1) - C) * vpic[d];
         const T xn = xp[d] + dt * vpic[d];
-        P.v[d][p] = vn[d];
-        P.x[par ^ 1][d][p] = xn;
-        if (!isfinite(vn[d]) || !isfinite(xn)) bad = true;
-        atomic_max_pos(&c->vmax[d], double(fabs(vn[d])));
+        if (live) {
+            P.v[d][p] = vn[d];
+            P.x[par ^ 1][d][p] = xn;
+            if (!isfinite(vn[d]) || !isfinite(xn)) bad = true;
+        }
+        block_max_pos(&c->vmax[d], live ? double(fabs(vn[d])) : 0.0, nullptr);
     }
     if (bad) atomicOr(&c->err, ERR_NONFINITE);
     if (M.musl) {
-        const T ms = mass_of(P, M, p, P.J[p]);
-        for_stencil<D, T>(g, w, [&](long no, T N, const T *, const int *) {
+        BlockAcc<D, D> acc;
+        acc.begin(smem, sint, live, w.base);
+        double *dst[D];
 #pragma unroll
-            for (int d = 0; d < D; d++) atomic_add(&G.pa[d][no], double(N * (ms * vn[d])));
+        for (int d = 0; d < D; d++) dst[d] = G.pa[d];
+        const T ms = mass_of(P, M, q, P.J[q]);
+        for_stencil<D, T>(g, w, [&](long no, T N, const T *, const int *ix) {
+            const int li = acc.local(ix);
+#pragma unroll
+            for (int d = 0; d < D; d++) acc.add(dst, d, no, li, double(N * (ms * vn[d])));
         });
+        acc.flush(dst, g);
     }
 }
 
@@ -540,14 +670,19 @@ __device__ __forceinline__ void constitutive(Ctrl *c, const PState<T> &P, const 
 // or (F-bar on) ΔJn scatter (volumetric.jl:1-11)
 template <int D, int B, typename T>
 __global__ void __launch_bounds__(256) k_deform(Ctrl *c, Geom g, PState<T> P, GState G, Mat M) {
+    __shared__ double smem[ACC_LDS];
+    __shared__ int sint[8];
+    if (!c->active) return;
     const long p = blockIdx.x * long(blockDim.x) + threadIdx.x;
-    if (p >= P.n || !c->active) return;
+    const bool live = p < P.n;
+    const long q = live ? p : 0;
     const int par = c->parity;
     T xp[D], lp[D];
 #pragma unroll
-    for (int i = 0; i < D; i++) { xp[i] = P.x[par][i][p]; lp[i] = B == 1 ? P.l0[i][p] : T(0); }
+    for (int i = 0; i < D; i++) { xp[i] = P.x[par][i][q]; lp[i] = B == 1 ? P.l0[i][q] : T(0); }
     W<D, T> w;
     weights<D, B, T>(g, xp, lp, w);
+    if (!live) w.ok = 0u;
     T gv[D][D];
 #pragma unroll
     for (int i = 0; i < D; i++)
@@ -569,7 +704,7 @@ __global__ void __launch_bounds__(256) k_deform(Ctrl *c, Geom g, PState<T> P, GS
 #pragma unroll
         for (int j = 0; j < D; j++) {
             dF[i][j] = (i == j ? T(1) : T(0)) + dt * gv[i][j];
-            F[i][j] = P.F[i + D * j][p];
+            F[i][j] = P.F[i + D * j][q];
         }
 #pragma unroll
     for (int i = 0; i < D; i++)
@@ -582,24 +717,34 @@ __global__ void __launch_bounds__(256) k_deform(Ctrl *c, Geom g, PState<T> P, GS
         }
     const T dJ = det<D, T>(dF);
     const T J = det<D, T>(Fn);
-#pragma unroll
-    for (int i = 0; i < D; i++)
-#pragma unroll
-        for (int j = 0; j < D; j++) {
-            P.F[i + D * j][p] = Fn[i][j];
-            if (M.tpic) P.gv[i + D * j][p] = gv[i][j];
-        }
-    P.J[p] = J;
-    if (M.locking) {
+    if (live) {
 #pragma unroll
         for (int i = 0; i < D; i++)
 #pragma unroll
-            for (int j = 0; j < D; j++) P.dF[i + D * j][p] = dF[i][j];
-        P.dJ[p] = dJ;
-        const T ms = mass_of(P, M, p, J);
-        const T q = ms * dJ;
-        for_stencil<D, T>(g, w, [&](long no, T N, const T *, const int *) { atomic_add(&G.dJ[no], double(N * q)); });
-    } else {
+            for (int j = 0; j < D; j++) {
+                P.F[i + D * j][p] = Fn[i][j];
+                if (M.tpic) P.gv[i + D * j][p] = gv[i][j];
+            }
+        P.J[p] = J;
+    }
+    if (M.locking) {
+        if (live) {
+#pragma unroll
+            for (int i = 0; i < D; i++)
+#pragma unroll
+                for (int j = 0; j < D; j++) P.dF[i + D * j][p] = dF[i][j];
+            P.dJ[p] = dJ;
+        }
+        BlockAcc<D, 1> acc;
+        acc.begin(smem, sint, live, w.base);
+        double *dst[1] = {G.dJ};
+        const T ms = mass_of(P, M, q, J);
+        const T qv = ms * dJ;
+        for_stencil<D, T>(g, w, [&](long no, T N, const T *, const int *ix) {
+            acc.add(dst, 0, no, acc.local(ix), double(N * qv));
+        });
+        acc.flush(dst, g);
+    } else if (live) {
         constitutive<D, T>(c, P, M, p, dF);
     }
 }

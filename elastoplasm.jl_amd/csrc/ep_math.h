@@ -5,7 +5,7 @@
 // same operation order, so the only differences from a sequential CPU run
 // are fused multiply-adds and the order of atomic accumulation.
 #pragma once
-#include <hip/hip_runtime.h>
+#include "ep_runtime.h"
 #include <stdint.h>
 
 namespace ep {

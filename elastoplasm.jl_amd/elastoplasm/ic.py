@@ -226,12 +226,12 @@ def get_slump(mesh, cmpr, instr, ni=2, lz=12.80):
 
 
 def get_column(mesh, cmpr, instr, ni=2, width=0.25, height=None):
-    """Rectangular column x <= width*L_x (and full height unless given):
-    the BASELINE column geometry, an addition to the reference's wedge."""
+    """Rectangular column x <= width*L_x, z <= height (default the domain
+    height L_z, i.e. below the 3D mesh's 2h headroom): the BASELINE column
+    geometry, an addition to the reference's wedge."""
     x = _lattice(mesh, ni)
     keep = x[0, :] <= width * mesh.L[0]
-    if height is not None:
-        keep &= x[-1, :] <= height
+    keep &= x[-1, :] <= (mesh.L[-1] if height is None else height)
     xp = np.asfortranarray(x[:, keep])
     nmp = xp.shape[1]
     return NT(xp=xp, coh0=np.full(nmp, cmpr.c0), cohr=np.full(nmp, cmpr.cr),
