@@ -1,9 +1,9 @@
 // ep_api.hip -- C ABI of libelastoplasm_hip.so (include/elastoplasm.h).
 //
 // The context owns one HIP stream, a control block in device memory and the
-// particle / grid buffers.  Upload sorts particles by background cell (the
-// reference's element numbering, tplgy.jl:43-96) so that neighbouring
-// threads touch neighbouring nodes; download undoes the permutation.
+// particle / grid buffers.  Upload sorts particles by background cell in
+// brick-major order (brick_key) so that a block of neighbouring threads
+// touches a compact box of nodes; download undoes the permutation.
 #include "../../include/elastoplasm.h"
 #include "ep_kernels.h"
 
@@ -88,6 +88,7 @@ struct ep_ctx {
     Ctrl *ctrl = nullptr;
     double *dt_log = nullptr;
     long dt_cap = 0;
+    unsigned long long *vmax_part = nullptr;
     Mat mat{};
     // profiling
     bool profile = false;
@@ -171,6 +172,7 @@ extern "C" void ep_destroy(ep_ctx *c) {
     for (void *a : c->part_allocs) hipFree(a);
     hipFree(c->ctrl);
     hipFree(c->dt_log);
+    hipFree(c->vmax_part);
     for (auto e : c->ev_pool) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -266,7 +268,11 @@ extern "C" int ep_upload_grid(ep_ctx *c, const ep_grid_view *gv) {
     return EP_OK;
 }
 
-static long elem_of(const ep_ctx *c, const double *x, long p) {
+// Sort key of a particle: its background cell in brick-major order.  Bricks
+// of 4x2x4 cells (3D, x-y-z) or 8x8 cells (2D) hold ~256 particles at the
+// reference's 2^d particles per cell, so a 256-thread block touches a
+// compact node box; inside a brick cells run z fastest, as the grid.
+static long brick_key(const ep_ctx *c, const double *x, long p) {
     const int D = c->dim;
     const Geom &g = c->geom;
     long e[3];
@@ -276,8 +282,14 @@ static long elem_of(const ep_ctx *c, const double *x, long p) {
         if (q > g.nel[i] - 1) q = g.nel[i] - 1;
         e[i] = long(q);
     }
-    if (D == 2) return e[1] + long(g.nel[1]) * e[0];
-    return e[2] + long(g.nel[2]) * (e[0] + long(g.nel[0]) * e[1]);
+    if (D == 2) {
+        const long nbz = (g.nel[1] + 7) / 8;
+        const long brick = e[1] / 8 + nbz * (e[0] / 8);
+        return brick * 64 + (e[1] % 8) + 8 * (e[0] % 8);
+    }
+    const long nbz = (g.nel[2] + 3) / 4, nbx = (g.nel[0] + 3) / 4;
+    const long brick = e[2] / 4 + nbz * (e[0] / 4 + nbx * (e[1] / 2));
+    return brick * 32 + (e[2] % 4) + 4 * ((e[0] % 4) + 4 * (e[1] % 2));
 }
 
 extern "C" int ep_upload_particles(ep_ctx *c, const ep_particles_view *v) {
@@ -294,7 +306,7 @@ extern "C" int ep_upload_particles(ep_ctx *c, const ep_particles_view *v) {
     if (M.tpic && !v->gradv) return fail(EP_E_ARG, "TPIC needs gradv");
     // cell-sorted order
     std::vector<long> key(n);
-    for (long p = 0; p < n; p++) key[p] = elem_of(c, v->x, p);
+    for (long p = 0; p < n; p++) key[p] = brick_key(c, v->x, p);
     c->perm.resize(n);
     std::iota(c->perm.begin(), c->perm.end(), 0L);
     std::stable_sort(c->perm.begin(), c->perm.end(), [&](long a, long b) { return key[a] < key[b]; });
@@ -374,6 +386,12 @@ extern "C" int ep_upload_particles(ep_ctx *c, const ep_particles_view *v) {
     }
     h.parity = 0;
     h.err = 0;
+    hipFree(c->vmax_part);
+    c->vmax_part = nullptr;
+    h.nblk = int((n + EP_BLOCK - 1) / EP_BLOCK);
+    HIPCHK(hipMalloc(&c->vmax_part, sizeof(unsigned long long) * 3 * size_t(h.nblk)));
+    HIPCHK(hipMemset(c->vmax_part, 0, sizeof(unsigned long long) * 3 * size_t(h.nblk)));
+    h.vmax_part = c->vmax_part;
     HIPCHK(hipMemcpy(c->ctrl, &h, sizeof(Ctrl), hipMemcpyHostToDevice));
     c->part_ready = true;
     return EP_OK;
@@ -525,7 +543,7 @@ static void launch_step(ep_ctx *c) {
     if (M.musl) timed(c, K_AUGV, [&] { EP_LAUNCH(k_augv<D>, gg, TB, s, ct, G); });
     timed(c, K_DEFORM, [&] { EP_LAUNCH((k_deform<D, B, double>), gp, TB, s, ct, g, P, G, M); });
     if (M.locking) timed(c, K_FBAR, [&] { EP_LAUNCH((k_fbar<D, B, double>), gp, TB, s, ct, g, P, G, M); });
-    timed(c, K_END, [&] { EP_LAUNCH(k_end, 1, 64, s, ct); });
+    timed(c, K_END, [&] { EP_LAUNCH(k_end, 1, EP_BLOCK, s, ct); });
 }
 
 static void dispatch_step(ep_ctx *c) {

@@ -59,6 +59,8 @@ struct Ctrl {
     int j2cap;
     double *dt_log;
     long dt_cap;
+    unsigned long long *vmax_part;  // [3][nblk] per-block maxima of |v_p| written by G2P
+    int nblk;
 };
 
 enum : unsigned { ERR_DOMAIN = 1u, ERR_NONFINITE = 2u };
@@ -185,22 +187,24 @@ __device__ __forceinline__ T mass_of(const PState<T> &P, const Mat &M, long p, T
 
 
 // -------------------------------------------------------------------------
-// Block scatter accumulator.  A block of 256 cell-sorted particles touches a
-// small box of nodes; the block reduces the box of its stencils, accumulates
-// its contributions in LDS (ds_add_f64) and flushes the box with one global
-// atomic per node value.  When the box does not fit the LDS budget (a block
-// spanning distant cells) contributions go straight to global atomics.
+// Block node box.  A block of 256 brick-sorted particles touches a small box
+// of nodes; the block reduces the box of its stencils and
+//  - for gathers, stages the nodal fields of the box in LDS with coalesced
+//    loads, so each particle reads its 4^d nodes from LDS;
+//  - for scatters, accumulates contributions in LDS (ds_add_f64) and flushes
+//    the box with one global atomic per node value.
+// When the box does not fit the LDS budget (a block spanning distant cells)
+// the kernels fall back to global loads / global atomics.
 // -------------------------------------------------------------------------
 constexpr int ACC_LDS = 4096;   // doubles of LDS per block (32 KiB)
 
-template <int D, int NV>
-struct BlockAcc {
+template <int D>
+struct BlockBox {
     int lo[D], ext[D];
     int nloc;
-    bool lds;
-    double *buf;
+    EP_LDS double *buf;
 
-    __device__ __forceinline__ void begin(double *smem, int *sint, bool live, const int *base) {
+    __device__ __forceinline__ void begin(EP_LDS double *smem, int *sint, bool live, const int *base) {
         buf = smem;
         if (threadIdx.x < D) { sint[threadIdx.x] = 0x7fffffff; sint[D + threadIdx.x] = -0x7fffffff; }
         __syncthreads();
@@ -209,50 +213,63 @@ struct BlockAcc {
             for (int i = 0; i < D; i++) { atomicMin(&sint[i], base[i]); atomicMax(&sint[D + i], base[i]); }
         }
         __syncthreads();
-        nloc = 1;
+        long n = 1;
+        bool ok = sint[0] != 0x7fffffff;
 #pragma unroll
         for (int i = 0; i < D; i++) {
             lo[i] = sint[i];
-            ext[i] = sint[D + i] - sint[i] + 4;
-            nloc *= ext[i];
+            const long e = long(sint[D + i]) - long(sint[i]) + 4;
+            ok = ok && e >= 4 && e <= ACC_LDS;
+            ext[i] = ok ? int(e) : 4;
+            n *= ext[i];
         }
-        if (sint[0] == 0x7fffffff) nloc = 0;
-        lds = nloc > 0 && nloc * NV <= ACC_LDS;
-        if (lds) {
-            for (int k = threadIdx.x; k < nloc * NV; k += blockDim.x) buf[k] = 0.0;
-        }
-        __syncthreads();
+        nloc = (ok && n <= ACC_LDS) ? int(n) : 0;
+        __syncthreads();   // everyone read sint before a later begin() rewrites it
     }
+    __device__ __forceinline__ bool fits(int nv) const { return nloc > 0 && nloc * nv <= ACC_LDS; }
     // local index: last axis (z) fastest, then x, then y, as the grid
     __device__ __forceinline__ int local(const int *ix) const {
         if (D == 2) return (ix[1] - lo[1]) + ext[1] * (ix[0] - lo[0]);
         return (ix[2] - lo[2]) + ext[2] * ((ix[0] - lo[0]) + ext[0] * (ix[1] - lo[1]));
     }
-    __device__ __forceinline__ void add(double *const *dst, int v, long no, int li, double val) {
-        if (lds) unsafeAtomicAdd(&buf[v * nloc + li], val);
-        else unsafeAtomicAdd(&dst[v][no], val);
+    // global node of local box entry k, or -1 outside the grid
+    __device__ __forceinline__ long node(const Geom &g, int k) const {
+        int ix[3];
+        if (D == 2) {
+            ix[1] = lo[1] + k % ext[1];
+            ix[0] = lo[0] + k / ext[1];
+            if (ix[0] < 0 || ix[1] < 0 || ix[0] >= g.nno[0] || ix[1] >= g.nno[1]) return -1;
+            return long(ix[1]) + long(g.nno[1]) * ix[0];
+        }
+        ix[2] = lo[2] + k % ext[2];
+        const int r = k / ext[2];
+        ix[0] = lo[0] + r % ext[0];
+        ix[1] = lo[1] + r / ext[0];
+        if (ix[0] < 0 || ix[1] < 0 || ix[2] < 0 || ix[0] >= g.nno[0] || ix[1] >= g.nno[1] || ix[2] >= g.nno[2])
+            return -1;
+        return long(ix[2]) + long(g.nno[2]) * (long(ix[0]) + long(g.nno[0]) * ix[1]);
     }
-    __device__ __forceinline__ void flush(double *const *dst, const Geom &g) {
-        if (!lds) return;
+    // LDS[v][k] = src[v][node(k)] (0 outside the grid); caller checks fits(nv)
+    template <int NV>
+    __device__ __forceinline__ void stage(const Geom &g, const double *const *src) {
+        for (int k = threadIdx.x; k < nloc; k += blockDim.x) {
+            const long no = node(g, k);
+#pragma unroll
+            for (int v = 0; v < NV; v++) buf[v * nloc + k] = no >= 0 ? src[v][no] : 0.0;
+        }
+        __syncthreads();
+    }
+    __device__ __forceinline__ void zero(int nv) {
+        __syncthreads();   // finished reading staged values
+        for (int k = threadIdx.x; k < nloc * nv; k += blockDim.x) buf[k] = 0.0;
+        __syncthreads();
+    }
+    template <int NV>
+    __device__ __forceinline__ void flush(const Geom &g, double *const *dst) {
         __syncthreads();
         for (int k = threadIdx.x; k < nloc; k += blockDim.x) {
-            int ix[3];
-            long no;
-            if (D == 2) {
-                ix[1] = lo[1] + k % ext[1];
-                ix[0] = lo[0] + k / ext[1];
-                if (ix[0] < 0 || ix[1] < 0 || ix[0] >= g.nno[0] || ix[1] >= g.nno[1]) continue;
-                no = long(ix[1]) + long(g.nno[1]) * ix[0];
-            } else {
-                ix[2] = lo[2] + k % ext[2];
-                const int r = k / ext[2];
-                ix[0] = lo[0] + r % ext[0];
-                ix[1] = lo[1] + r / ext[0];
-                if (ix[0] < 0 || ix[1] < 0 || ix[2] < 0 || ix[0] >= g.nno[0] || ix[1] >= g.nno[1] ||
-                    ix[2] >= g.nno[2])
-                    continue;
-                no = long(ix[2]) + long(g.nno[2]) * (long(ix[0]) + long(g.nno[0]) * ix[1]);
-            }
+            const long no = node(g, k);
+            if (no < 0) continue;
 #pragma unroll
             for (int v = 0; v < NV; v++) {
                 const double val = buf[v * nloc + k];
@@ -262,16 +279,116 @@ struct BlockAcc {
     }
 };
 
-// Block max of non-negative doubles into a global bit-pattern max.
-__device__ __forceinline__ void block_max_pos(unsigned long long *dst, double v, unsigned *sred) {
-    unsigned long long b = (unsigned long long)__double_as_longlong(v);
+// Scatter targets: the block's LDS box or the global nodal arrays.  Each
+// scatter loop is instantiated once per sink, so LDS traffic stays ds_*.
+template <int D>
+struct LdsSink {
+    BlockBox<D> *b;
+    __device__ __forceinline__ int local(const int *ix) const { return b->local(ix); }
+    __device__ __forceinline__ void operator()(int v, long, int li, double val) const {
+        lds_atomic_add(&b->buf[v * b->nloc + li], val);
+    }
+};
+struct GlobSink {
+    double *const *dst;
+    __device__ __forceinline__ int local(const int *) const { return 0; }
+    __device__ __forceinline__ void operator()(int v, long no, int, double val) const {
+        unsafeAtomicAdd(&dst[v][no], val);
+    }
+};
+// Gather sources: the staged LDS box or the global nodal arrays.
+template <int D>
+struct LdsSrc {
+    const BlockBox<D> *b;
+    __device__ __forceinline__ int local(const int *ix) const { return b->local(ix); }
+    __device__ __forceinline__ double operator()(int v, long, int li) const { return b->buf[v * b->nloc + li]; }
+};
+struct GlobSrc {
+    const double *const *src;
+    __device__ __forceinline__ int local(const int *) const { return 0; }
+    __device__ __forceinline__ double operator()(int v, long no, int) const { return src[v][no]; }
+};
+
+template <typename T>
+__device__ __forceinline__ T sel4(const T *a, int i) {
+    return i == 0 ? a[0] : (i == 1 ? a[1] : (i == 2 ? a[2] : a[3]));
+}
+
+// Scatter iteration over the 4^d stencil with a lane-dependent rotation of
+// the x and z offsets: the particles of one cell sit in consecutive lanes,
+// and the rotation sends them to different nodes at the same instant, so
+// their LDS atomics do not collide.  Scatter order does not matter (the
+// reference accumulates with atomics too); gathers keep for_stencil's
+// reference order.
+template <int D, typename T, typename Fn>
+__device__ __forceinline__ void for_stencil_rot(const Geom &g, const W<D, T> &w, Fn &&fn) {
+    const int lane = threadIdx.x & 63;
+    if (D == 2) {
+        const int rz = lane & 3;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            if (!(w.ok >> i & 1u)) continue;
+#pragma unroll
+            for (int j0 = 0; j0 < 4; j0++) {
+                const int j = (j0 + rz) & 3;
+                if (!(w.ok >> (4 + j) & 1u)) continue;
+                const int ix[2] = {w.base[0] + i, w.base[1] + j};
+                const long no = long(ix[1]) + long(g.nno[1]) * long(ix[0]);
+                const T wz = sel4(w.w[1], j), dwz = sel4(w.dw[1], j);
+                T dN[D];
+                const T N = w.w[0][i] * wz;
+                dN[0] = w.dw[0][i] * wz;
+                dN[1] = w.w[0][i] * dwz;
+                fn(no, N, dN, ix);
+            }
+        }
+    } else {
+        const int rz = lane & 3, rx = (lane >> 2) & 1;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (!(w.ok >> (4 + k) & 1u)) continue;
+#pragma unroll
+            for (int i0 = 0; i0 < 4; i0++) {
+                const int i = (i0 + rx) & 3;
+                if (!(w.ok >> i & 1u)) continue;
+                const T wx = sel4(w.w[0], i), dwx = sel4(w.dw[0], i);
+                const T wxy = wx * w.w[1][k];
+                const T dxy = dwx * w.w[1][k];
+                const T xdy = wx * w.dw[1][k];
+#pragma unroll
+                for (int j0 = 0; j0 < 4; j0++) {
+                    const int j = (j0 + rz) & 3;
+                    if (!(w.ok >> (8 + j) & 1u)) continue;
+                    const int ix[3] = {w.base[0] + i, w.base[1] + k, w.base[2] + j};
+                    const long no = long(ix[2]) + long(g.nno[2]) * (long(ix[0]) + long(g.nno[0]) * long(ix[1]));
+                    const T wz = sel4(w.w[2], j), dwz = sel4(w.dw[2], j);
+                    T dN[D];
+                    const T N = wxy * wz;
+                    dN[0] = dxy * wz;
+                    dN[1] = xdy * wz;
+                    dN[2] = wxy * dwz;
+                    fn(no, N, dN, ix);
+                }
+            }
+        }
+    }
+}
+
+// Block max of non-negative doubles (as ordered bit patterns): wavefront
+// butterfly, then across the block's waves through LDS; thread 0 returns it.
+__device__ __forceinline__ unsigned long long block_max_bits(unsigned long long b, unsigned long long *sred) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
         const unsigned long long o = __shfl_xor(b, off);
         b = o > b ? o : b;
     }
-    (void)sred;
-    if ((threadIdx.x & 63) == 0) atomicMax(dst, b);
+    const int wv = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    if ((threadIdx.x & 63) == 0) sred[wv] = b;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int k = 1; k < nw; k++) b = sred[k] > b ? sred[k] : b;
+    __syncthreads();
+    return b;
 }
 
 template <int D, typename T>
@@ -310,7 +427,6 @@ __global__ void k_begin(Ctrl *c, Geom g, Mat M) {
     } else {
         dt = c->dt_fixed;
     }
-    for (int i = 0; i < D; i++) c->vmax[i] = 0ull;
     double G = c->G;
     if (c->ramp) G = (t <= c->tg) ? (c->G * t / c->tg) : c->G;
     for (int i = 0; i < 3; i++) c->g[i] = 0.0;
@@ -319,8 +435,21 @@ __global__ void k_begin(Ctrl *c, Geom g, Mat M) {
     if (c->record && c->steps < c->dt_cap) c->dt_log[c->steps] = dt;
 }
 
+// End of step: vmax = max over G2P's per-block maxima (the atomic max of
+// n2p.jl:37), then time.t[1] += dt (worflow.jl:38, 84).
 __global__ void k_end(Ctrl *c) {
-    if (threadIdx.x != 0 || !c->active) return;
+    __shared__ unsigned long long sred[16];
+    if (!c->active) return;
+    for (int d = 0; d < 3; d++) {
+        unsigned long long b = 0ull;
+        for (int k = threadIdx.x; k < c->nblk; k += blockDim.x) {
+            const unsigned long long o = c->vmax_part[d * c->nblk + k];
+            b = o > b ? o : b;
+        }
+        b = block_max_bits(b, sred);
+        if (threadIdx.x == 0) c->vmax[d] = b;
+    }
+    if (threadIdx.x != 0) return;
     c->t = c->t + c->dt;
     c->it += 1;
     c->steps += 1;
@@ -356,8 +485,10 @@ __global__ void __launch_bounds__(256) k_p2g(const Ctrl *c, Geom g, PState<T> P,
     W<D, T> w;
     if (!weights<D, B, T>(g, xp, lp, w) && live) atomicOr(const_cast<unsigned *>(&c->err), ERR_DOMAIN);
     if (!live) w.ok = 0u;
-    BlockAcc<D, NV> acc;
-    acc.begin(smem, sint, live, w.base);
+    BlockBox<D> box;
+    box.begin((EP_LDS double *)smem, sint, live, w.base);
+    const bool lds = box.fits(NV);
+    if (lds) box.zero(NV);
     double *dst[NV];
     dst[0] = G.m;
 #pragma unroll
@@ -379,42 +510,46 @@ __global__ void __launch_bounds__(256) k_p2g(const Ctrl *c, Geom g, PState<T> P,
     }
     const T gz = T(c->g[D - 1]);
     const bool tpic = M.tpic;
-    for_stencil<D, T>(g, w, [&](long no, T N, const T *dN, const int *ix) {
-        const int li = acc.local(ix);
-        acc.add(dst, 0, no, li, double(N * ms));
+    auto scatter = [&](const auto &sink) {
+    for_stencil_rot<D, T>(g, w, [&](long no, T N, const T *dN, const int *ix) {
+        const int li = sink.local(ix);
+        sink(0, no, li, double(N * ms));
         if (D == 2) {
             if (tpic) {
                 // Δnp = x_n - x_p (misc.jl:1-14)
                 const T dx = T(g.X[0][ix[0]]) - xp[0];
                 const T dy = T(g.X[1][ix[1]]) - xp[1];
-                acc.add(dst, 1, no, li, double(N * ms * (v[0] + gv[0][0] * dx + gv[0][1] * dy)));
-                acc.add(dst, 2, no, li, double(N * ms * (v[1] + gv[1][0] * dx + gv[1][1] * dy)));
+                sink(1, no, li, double(N * ms * (v[0] + gv[0][0] * dx + gv[0][1] * dy)));
+                sink(2, no, li, double(N * ms * (v[1] + gv[1][0] * dx + gv[1][1] * dy)));
             } else {
-                acc.add(dst, 1, no, li, double(N * pm[0]));
-                acc.add(dst, 2, no, li, double(N * pm[1]));
+                sink(1, no, li, double(N * pm[0]));
+                sink(2, no, li, double(N * pm[1]));
             }
-            acc.add(dst, 3, no, li, double(-(Om * (dN[0] * s[0] + dN[1] * s[2]))));
-            acc.add(dst, 4, no, li, double(-(Om * (dN[0] * s[2] + dN[1] * s[1]) - N * (ms * gz))));
+            sink(3, no, li, double(-(Om * (dN[0] * s[0] + dN[1] * s[2]))));
+            sink(4, no, li, double(-(Om * (dN[0] * s[2] + dN[1] * s[1]) - N * (ms * gz))));
         } else {
             if (tpic) {
                 const T dx = T(g.X[0][ix[0]]) - xp[0];
                 const T dy = T(g.X[1][ix[1]]) - xp[1];
                 const T dz = T(g.X[2][ix[2]]) - xp[2];
-                acc.add(dst, 1, no, li, double(N * ms * (v[0] + gv[0][0] * dx + gv[0][1] * dy + gv[0][2] * dz)));
-                acc.add(dst, 2, no, li, double(N * ms * (v[1] + gv[1][0] * dx + gv[1][1] * dy + gv[1][2] * dz)));
-                acc.add(dst, 3, no, li, double(N * ms * (v[2] + gv[2][0] * dx + gv[2][1] * dy + gv[2][2] * dz)));
+                sink(1, no, li, double(N * ms * (v[0] + gv[0][0] * dx + gv[0][1] * dy + gv[0][2] * dz)));
+                sink(2, no, li, double(N * ms * (v[1] + gv[1][0] * dx + gv[1][1] * dy + gv[1][2] * dz)));
+                sink(3, no, li, double(N * ms * (v[2] + gv[2][0] * dx + gv[2][1] * dy + gv[2][2] * dz)));
             } else {
-                acc.add(dst, 1, no, li, double(N * pm[0]));
-                acc.add(dst, 2, no, li, double(N * pm[1]));
-                acc.add(dst, 3, no, li, double(N * pm[2]));
+                sink(1, no, li, double(N * pm[0]));
+                sink(2, no, li, double(N * pm[1]));
+                sink(3, no, li, double(N * pm[2]));
             }
             // Voigt [xx,yy,zz,yz,xz,xy]: σyx = s[5], σzy = s[3], σzx = s[4]
-            acc.add(dst, 4, no, li, double(-(Om * (dN[0] * s[0] + dN[1] * s[5] + dN[2] * s[4]))));
-            acc.add(dst, 5, no, li, double(-(Om * (dN[0] * s[5] + dN[1] * s[1] + dN[2] * s[3]))));
-            acc.add(dst, 6, no, li, double(-(Om * (dN[0] * s[4] + dN[1] * s[3] + dN[2] * s[2]) - N * (ms * gz))));
+            sink(4, no, li, double(-(Om * (dN[0] * s[0] + dN[1] * s[5] + dN[2] * s[4]))));
+            sink(5, no, li, double(-(Om * (dN[0] * s[5] + dN[1] * s[1] + dN[2] * s[3]))));
+            sink(6, no, li, double(-(Om * (dN[0] * s[4] + dN[1] * s[3] + dN[2] * s[2]) - N * (ms * gz))));
         }
     });
-    acc.flush(dst, g);
+    };
+    if (lds) scatter(LdsSink<D>{&box});
+    else scatter(GlobSink{dst});
+    if (lds) box.template flush<NV>(g, dst);
 }
 
 // euler (solve.jl:14-40); a and v of massless nodes are the zeros that
@@ -460,6 +595,7 @@ template <int D, int B, typename T>
 __global__ void __launch_bounds__(256) k_g2p(Ctrl *c, Geom g, PState<T> P, GState G, Mat M) {
     __shared__ double smem[ACC_LDS];
     __shared__ int sint[8];
+    __shared__ unsigned long long sred[16];
     if (!c->active) return;
     const long p = blockIdx.x * long(blockDim.x) + threadIdx.x;
     const bool live = p < P.n;
@@ -471,16 +607,33 @@ __global__ void __launch_bounds__(256) k_g2p(Ctrl *c, Geom g, PState<T> P, GStat
     W<D, T> w;
     weights<D, B, T>(g, xp, lp, w);
     if (!live) w.ok = 0u;
+    BlockBox<D> box;
+    box.begin((EP_LDS double *)smem, sint, live, w.base);
+    const bool staged = box.fits(2 * D);
+    if (staged) {
+        const double *src[2 * D];
+#pragma unroll
+        for (int d = 0; d < D; d++) { src[d] = G.v[d]; src[D + d] = G.a[d]; }
+        box.template stage<2 * D>(g, src);
+    }
     T vpic[D], aflip[D];
 #pragma unroll
     for (int d = 0; d < D; d++) { vpic[d] = T(0); aflip[d] = T(0); }
-    for_stencil<D, T>(g, w, [&](long no, T N, const T *, const int *) {
+    const double *gsrc[2 * D];
 #pragma unroll
-        for (int d = 0; d < D; d++) {
-            vpic[d] += N * T(G.v[d][no]);
-            aflip[d] += N * T(G.a[d][no]);
-        }
-    });
+    for (int d = 0; d < D; d++) { gsrc[d] = G.v[d]; gsrc[D + d] = G.a[d]; }
+    auto gather = [&](const auto &src) {
+        for_stencil<D, T>(g, w, [&](long no, T N, const T *, const int *ix) {
+            const int li = src.local(ix);
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                vpic[d] += N * T(src(d, no, li));
+                aflip[d] += N * T(src(D + d, no, li));
+            }
+        });
+    };
+    if (staged) gather(LdsSrc<D>{&box});
+    else gather(GlobSrc{gsrc});
     const T dt = T(c->dt), C = T(M.C_pf);
     T vn[D];
     bool bad = false;
@@ -493,22 +646,28 @@ __global__ void __launch_bounds__(256) k_g2p(Ctrl *c, Geom g, PState<T> P, GStat
             P.x[par ^ 1][d][p] = xn;
             if (!isfinite(vn[d]) || !isfinite(xn)) bad = true;
         }
-        block_max_pos(&c->vmax[d], live ? double(fabs(vn[d])) : 0.0, nullptr);
+        const unsigned long long bm =
+            block_max_bits(live ? (unsigned long long)__double_as_longlong(double(fabs(vn[d]))) : 0ull, sred);
+        if (threadIdx.x == 0) c->vmax_part[d * c->nblk + blockIdx.x] = bm;
     }
     if (bad) atomicOr(&c->err, ERR_NONFINITE);
     if (M.musl) {
-        BlockAcc<D, D> acc;
-        acc.begin(smem, sint, live, w.base);
+        const bool lds = box.fits(D);
+        if (lds) box.zero(D);
         double *dst[D];
 #pragma unroll
         for (int d = 0; d < D; d++) dst[d] = G.pa[d];
         const T ms = mass_of(P, M, q, P.J[q]);
-        for_stencil<D, T>(g, w, [&](long no, T N, const T *, const int *ix) {
-            const int li = acc.local(ix);
+        auto scatter = [&](const auto &sink) {
+            for_stencil_rot<D, T>(g, w, [&](long no, T N, const T *, const int *ix) {
+                const int li = sink.local(ix);
 #pragma unroll
-            for (int d = 0; d < D; d++) acc.add(dst, d, no, li, double(N * (ms * vn[d])));
-        });
-        acc.flush(dst, g);
+                for (int d = 0; d < D; d++) sink(d, no, li, double(N * (ms * vn[d])));
+            });
+        };
+        if (lds) scatter(LdsSink<D>{&box});
+        else scatter(GlobSink{dst});
+        if (lds) box.template flush<D>(g, dst);
     }
 }
 
@@ -683,20 +842,29 @@ __global__ void __launch_bounds__(256) k_deform(Ctrl *c, Geom g, PState<T> P, GS
     W<D, T> w;
     weights<D, B, T>(g, xp, lp, w);
     if (!live) w.ok = 0u;
+    BlockBox<D> box;
+    box.begin((EP_LDS double *)smem, sint, live, w.base);
+    const bool staged = box.fits(D);
+    if (staged) box.template stage<D>(g, G.v);
     T gv[D][D];
 #pragma unroll
     for (int i = 0; i < D; i++)
 #pragma unroll
         for (int j = 0; j < D; j++) gv[i][j] = T(0);
-    for_stencil<D, T>(g, w, [&](long no, T, const T *dN, const int *) {
-        T vn[D];
+    auto gather = [&](const auto &src) {
+        for_stencil<D, T>(g, w, [&](long no, T, const T *dN, const int *ix) {
+            const int li = src.local(ix);
+            T vn[D];
 #pragma unroll
-        for (int i = 0; i < D; i++) vn[i] = T(G.v[i][no]);
+            for (int i = 0; i < D; i++) vn[i] = T(src(i, no, li));
 #pragma unroll
-        for (int i = 0; i < D; i++)
+            for (int i = 0; i < D; i++)
 #pragma unroll
-            for (int j = 0; j < D; j++) gv[i][j] += dN[j] * vn[i];
-    });
+                for (int j = 0; j < D; j++) gv[i][j] += dN[j] * vn[i];
+        });
+    };
+    if (staged) gather(LdsSrc<D>{&box});
+    else gather(GlobSrc{G.v});
     const T dt = T(c->dt);
     T dF[D][D], F[D][D], Fn[D][D];
 #pragma unroll
@@ -735,38 +903,69 @@ __global__ void __launch_bounds__(256) k_deform(Ctrl *c, Geom g, PState<T> P, GS
                 for (int j = 0; j < D; j++) P.dF[i + D * j][p] = dF[i][j];
             P.dJ[p] = dJ;
         }
-        BlockAcc<D, 1> acc;
-        acc.begin(smem, sint, live, w.base);
+        const bool lds = box.fits(1);
+        if (lds) box.zero(1);
         double *dst[1] = {G.dJ};
         const T ms = mass_of(P, M, q, J);
         const T qv = ms * dJ;
-        for_stencil<D, T>(g, w, [&](long no, T N, const T *, const int *ix) {
-            acc.add(dst, 0, no, acc.local(ix), double(N * qv));
-        });
-        acc.flush(dst, g);
+        auto scatter = [&](const auto &sink) {
+            for_stencil_rot<D, T>(g, w, [&](long no, T N, const T *, const int *ix) {
+                sink(0, no, sink.local(ix), double(N * qv));
+            });
+        };
+        if (lds) scatter(LdsSink<D>{&box});
+        else scatter(GlobSink{dst});
+        if (lds) box.template flush<1>(g, dst);
     } else if (live) {
         constitutive<D, T>(c, P, M, p, dF);
     }
 }
 
-// ΔJs + ΔJp (volumetric.jl:12-35) fused, then elast/plast
+// ΔJs + ΔJp (volumetric.jl:12-35) fused, then elast/plast.  The staged
+// nodal value is ΔJ_n/m_n exactly as ΔJs writes it (0 where m_n = 0).
 template <int D, int B, typename T>
 __global__ void __launch_bounds__(256) k_fbar(Ctrl *c, Geom g, PState<T> P, GState G, Mat M) {
+    __shared__ double smem[ACC_LDS];
+    __shared__ int sint[8];
+    if (!c->active) return;
     const long p = blockIdx.x * long(blockDim.x) + threadIdx.x;
-    if (p >= P.n || !c->active) return;
+    const bool live = p < P.n;
+    const long q = live ? p : 0;
     const int par = c->parity;
     T xp[D], lp[D];
 #pragma unroll
-    for (int i = 0; i < D; i++) { xp[i] = P.x[par][i][p]; lp[i] = B == 1 ? P.l0[i][p] : T(0); }
+    for (int i = 0; i < D; i++) { xp[i] = P.x[par][i][q]; lp[i] = B == 1 ? P.l0[i][q] : T(0); }
     W<D, T> w;
     weights<D, B, T>(g, xp, lp, w);
+    if (!live) w.ok = 0u;
+    BlockBox<D> box;
+    box.begin((EP_LDS double *)smem, sint, live, w.base);
+    const bool staged = box.fits(1);
+    if (staged) {
+        for (int k = threadIdx.x; k < box.nloc; k += blockDim.x) {
+            const long no = box.node(g, k);
+            double val = 0.0;
+            if (no >= 0) {
+                const double m = G.m[no];
+                val = m == 0.0 ? 0.0 : G.dJ[no] / m;
+            }
+            box.buf[k] = val;
+        }
+        __syncthreads();
+    }
+    if (!live) return;
     const T dJp = P.dJ[p];
     T acc = T(0);
-    for_stencil<D, T>(g, w, [&](long no, T N, const T *, const int *) {
-        const double m = G.m[no];
-        const T q = m == 0.0 ? T(0) : T(G.dJ[no] / m);
-        acc += N * q / dJp;
-    });
+    if (staged) {
+        for_stencil<D, T>(g, w, [&](long, T N, const T *, const int *ix) {
+            acc += N * T(box.buf[box.local(ix)]) / dJp;
+        });
+    } else {
+        for_stencil<D, T>(g, w, [&](long no, T N, const T *, const int *) {
+            const double m = G.m[no];
+            acc += N * (m == 0.0 ? T(0) : T(G.dJ[no] / m)) / dJp;
+        });
+    }
     const T f = pow(acc, T(1.0 / D));
     T dF[D][D];
 #pragma unroll
