@@ -15,6 +15,7 @@ on the host cores over a bounded sample of the same workload).
 from __future__ import annotations
 
 import argparse
+import copy
 import json
 import math
 import os
@@ -115,24 +116,33 @@ def main():
     ic, cfg = build_workload(args.workload)
     nmp = ic.mpts.nmp
     dim = ic.mesh.dim
-    dev = hip.Device(ic.mesh, ic.mpts, ic.cmpr, cfg.instr)
     print(f"[bench] {args.workload}: nmp={nmp} nno={int(ic.mesh.nno[-1])} setup {time.perf_counter()-t_setup:.1f}s",
           file=sys.stderr, flush=True)
-    T = 1e9
-    if args.warmup:
-        dev.step(args.warmup, T)
-    dev.profile(True)
-    t0 = time.perf_counter()
-    r = dev.step(args.steps, T)
-    el = time.perf_counter() - t0
-    ktimes = dev.kernel_times()
-    dev.profile(False)
-    assert r.steps == args.steps, "steps were skipped"
+
+    def timed_run(precision):
+        """warmup + timed steps from the initial state; returns (pu/s, s,
+        per-kernel {name: ms per launch}, final state)"""
+        st = copy.deepcopy(ic)
+        instr = copy.deepcopy(cfg.instr)
+        instr["ops"]["precision"] = {"fp64": 0, "fp32": 1}[precision]
+        dev = hip.Device(st.mesh, st.mpts, st.cmpr, instr)
+        if args.warmup:
+            dev.step(args.warmup, 1e9)
+        dev.profile(True)
+        t0 = time.perf_counter()
+        r = dev.step(args.steps, 1e9)
+        el = time.perf_counter() - t0
+        kt = dev.kernel_times()
+        dev.profile(False)
+        assert r.steps == args.steps, "steps were skipped"
+        dev.download()
+        dev.close()
+        return nmp * args.steps / el, el, {k: ms / n for k, (ms, n) in kt.items()}, st
+
+    value, el, per_kernel, st64 = timed_run("fp64")
     ms_step = 1000.0 * el / args.steps
-    value = nmp * args.steps / el
     print(f"[bench] {args.steps} steps in {el:.3f}s: {ms_step:.3f} ms/step, {value:.4g} pu/s", file=sys.stderr,
           flush=True)
-    per_kernel = {k: ms / n for k, (ms, n) in ktimes.items()}
     for k, v in sorted(per_kernel.items(), key=lambda kv: -kv[1]):
         print(f"[bench]   {k:10s} {v*1000:9.1f} us/launch", file=sys.stderr)
     dom = max((k for k in per_kernel if k in KERNEL_BYTES_3D), key=lambda k: per_kernel[k])
@@ -144,9 +154,15 @@ def main():
             "kernel_ms": per_kernel[dom], "bytes_per_particle": kb}
     step_roof = {"bytes_per_particle": B_ALG[dim], "achieved": B_ALG[dim] * value / 1e9,
                  "frac": B_ALG[dim] * value / 1e9 / HBM_PEAK_GBS, "kernel_ms_per_step": step_ms}
-    dev.download()
-    finite = bool(np.isfinite(ic.mpts.s.tau).all())
-    dev.close()
+    finite = bool(np.isfinite(st64.mpts.s.tau).all())
+    v32, el32, pk32, st32 = timed_run("fp32")
+    err32 = float(np.linalg.norm(st32.mpts.s.tau - st64.mpts.s.tau) / np.linalg.norm(st64.mpts.s.tau))
+    x0 = ic.mpts.x
+    errx32 = float(np.linalg.norm(st32.mpts.x - st64.mpts.x) / np.linalg.norm(st64.mpts.x - x0))
+    print(f"[bench] fp32: {1000*el32/args.steps:.3f} ms/step, {v32:.4g} pu/s, stress err vs fp64 {err32:.2e}",
+          file=sys.stderr, flush=True)
+    for k, v in sorted(pk32.items(), key=lambda kv: -kv[1]):
+        print(f"[bench]   fp32 {k:10s} {v*1000:9.1f} us/launch", file=sys.stderr)
     cpu = None
     if not args.no_cpu_baseline:
         thr = min(16, os.cpu_count() or 1)
@@ -170,6 +186,10 @@ def main():
         "roofline": roof,
         "step_roofline": step_roof,
         "kernels_us": {k: v * 1000 for k, v in per_kernel.items()},
+        "fp32": {"value": v32, "ms_per_step": 1000.0 * el32 / args.steps, "speedup_vs_fp64": v32 / value,
+                 "rel_stress_err_vs_fp64": err32, "rel_displacement_err_vs_fp64": errx32,
+                 "kernels_us": {k: v * 1000 for k, v in pk32.items()},
+                 "dtype": "f32 particles, f64 grid and near-degenerate eigen-decompositions"},
         "state_finite": finite,
         "cpu_baseline": cpu,
     }

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -78,7 +79,9 @@ struct ep_ctx {
     uint8_t *bc_dev = nullptr;
     bool grid_ready = false;
     // particles
-    PState<double> ps{};
+    PState<double> ps{};               // FP64 particle state (EP_FP64)
+    PState<float> psf{};               // FP32 particle state (EP_FP32, EP_FP32_SURROGATE)
+    bool fp32 = false;
     std::vector<void *> part_allocs;   // one allocation per particle field
     long nmp = 0;
     std::vector<long> perm;          // device slot -> host particle index
@@ -115,7 +118,8 @@ extern "C" int ep_create(const ep_config *cfg, ep_ctx **out) {
     if (cfg->deform != EP_FINITE && cfg->deform != EP_INFINITESIMAL)
         return fail(EP_E_CONFIG, "unsupported deformation framework");
     if (cfg->plast < EP_PLAST_NONE || cfg->plast > EP_J2) return fail(EP_E_CONFIG, "InvalidReturnMapping");
-    if (cfg->precision != EP_FP64) return fail(EP_E_CONFIG, "precision mode not available in this build");
+    if (cfg->precision != EP_FP64 && cfg->precision != EP_FP32 && cfg->precision != EP_FP32_SURROGATE)
+        return fail(EP_E_CONFIG, "unknown precision mode");
     if (cfg->nmp <= 0) return fail(EP_E_ARG, "nmp must be positive");
     for (int i = 0; i < cfg->dim; i++) {
         if (cfg->nel[i] < 1 || cfg->nno[i] < 2 || cfg->nno[i] > (1 << 24) || !(cfg->h[i] > 0.0))
@@ -123,6 +127,7 @@ extern "C" int ep_create(const ep_config *cfg, ep_ctx **out) {
     }
     ep_ctx *c = new ep_ctx();
     c->cfg = *cfg;
+    c->fp32 = cfg->precision != EP_FP64;
     c->dim = cfg->dim;
     c->nstr = cfg->dim == 2 ? 3 : 6;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -145,6 +150,7 @@ extern "C" int ep_create(const ep_config *cfg, ep_ctx **out) {
         g.nno[i] = i < c->dim ? int(cfg->nno[i]) : 1;
         g.nel[i] = i < c->dim ? int(cfg->nel[i]) : 1;
         g.h[i] = i < c->dim ? cfg->h[i] : 1.0;
+        g.ih[i] = 1.0 / g.h[i];
         g.x0[i] = i < c->dim ? cfg->x0[i] : 0.0;
         nno_tot *= g.nno[i];
     }
@@ -292,6 +298,83 @@ static long brick_key(const ep_ctx *c, const double *x, long p) {
     return brick * 32 + (e[2] % 4) + 4 * ((e[0] % 4) + 4 * (e[1] % 2));
 }
 
+// Particle fields in device order, one allocation per field, stored as T.
+template <typename T>
+static int upload_fields(ep_ctx *c, const ep_particles_view *v, PState<T> &P) {
+    const int D = c->dim, NS = c->nstr, DD = D * D;
+    const long n = c->cfg.nmp;
+    const Mat &M = c->mat;
+    // one device allocation per particle field
+    for (void *a : c->part_allocs) hipFree(a);
+    c->part_allocs.clear();
+    c->nmp = n;
+    P = PState<T>{};
+    bool oom = false;
+    auto take = [&]() -> T * {
+        void *a = nullptr;
+        if (hipMalloc(&a, sizeof(T) * n) != hipSuccess) { oom = true; return nullptr; }
+        c->part_allocs.push_back(a);
+        return static_cast<T *>(a);
+    };
+    auto take_d = [&]() -> double * {
+        void *a = nullptr;
+        if (hipMalloc(&a, sizeof(double) * n) != hipSuccess) { oom = true; return nullptr; }
+        c->part_allocs.push_back(a);
+        return static_cast<double *>(a);
+    };
+    for (int k = 0; k < 2; k++) for (int i = 0; i < D; i++) P.x[k][i] = take_d();
+    for (int i = 0; i < D; i++) P.v[i] = take();
+    for (int i = 0; i < DD; i++) P.F[i] = take();
+    for (int i = 0; i < (D == 2 ? 3 : 6); i++) P.b[i] = take();
+    for (int i = 0; i < NS; i++) P.s[i] = take();
+    P.epII = take(); P.dlam = take(); P.J = take(); P.rho = take();
+    P.Om0 = take(); P.c0 = take(); P.cr = take(); P.phi = take();
+    if (c->cfg.basis == EP_GIMPM) for (int i = 0; i < D; i++) P.l0[i] = take();
+    if (M.tpic) for (int i = 0; i < DD; i++) P.gv[i] = take();
+    if (oom) return fail(EP_E_HIP, "hipMalloc of particle fields failed");
+    P.n = n;
+    // host staging in device order
+    auto put = [&](auto *dst, auto getter) -> int {
+        using E = std::remove_pointer_t<decltype(dst)>;
+        std::vector<E> st(n);
+        for (long q = 0; q < n; q++) st[q] = E(getter(c->perm[q]));
+        hipError_t e = hipMemcpy(dst, st.data(), sizeof(E) * n, hipMemcpyHostToDevice);
+        return e == hipSuccess ? EP_OK : fail(EP_E_HIP, "hipMemcpy particle upload failed");
+    };
+    int r;
+#define PUT(dst, expr) do { if ((r = put(dst, [&](long p) { return (expr); })) != EP_OK) return r; } while (0)
+    for (int i = 0; i < D; i++) { PUT(P.x[0][i], v->x[i + D * p]); PUT(P.x[1][i], v->x[i + D * p]); }
+    for (int i = 0; i < D; i++) PUT(P.v[i], v->v[i + D * p]);
+    for (int i = 0; i < DD; i++) PUT(P.F[i], v->F[i + DD * p]);
+    if (M.finite) {
+        // symmetric part of bᵢⱼ: 2D xx,yy,xy; 3D xx,yy,zz,yz,xz,xy
+        static const int i2[3][2] = {{0, 0}, {1, 1}, {0, 1}};
+        static const int i3[6][2] = {{0, 0}, {1, 1}, {2, 2}, {1, 2}, {0, 2}, {0, 1}};
+        for (int k = 0; k < (D == 2 ? 3 : 6); k++) {
+            const int a = D == 2 ? i2[k][0] : i3[k][0], bb = D == 2 ? i2[k][1] : i3[k][1];
+            // the FP32 path stores B = b - I
+            const double off = (c->fp32 && a == bb) ? 1.0 : 0.0;
+            PUT(P.b[k], 0.5 * (v->b[a + D * bb + DD * p] + v->b[bb + D * a + DD * p]) - off);
+        }
+        for (int k = 0; k < NS; k++) PUT(P.s[k], v->tau[k + NS * p]);
+    } else {
+        for (int k = 0; k < (D == 2 ? 3 : 6); k++) PUT(P.b[k], (k < D && !c->fp32) ? 1.0 : 0.0);
+        for (int k = 0; k < NS; k++) PUT(P.s[k], v->sig[k + NS * p]);
+    }
+    PUT(P.epII, v->epII[0 + 2 * p]);
+    PUT(P.dlam, v->dlam ? v->dlam[p] : 0.0);
+    PUT(P.J, v->J[p]);
+    PUT(P.rho, M.finite ? v->rho0[p] : v->rho[p]);
+    PUT(P.Om0, v->Om0[p]);
+    PUT(P.c0, v->c0[p]);
+    PUT(P.cr, v->cr[p]);
+    PUT(P.phi, v->phi[p]);
+    if (c->cfg.basis == EP_GIMPM) for (int i = 0; i < D; i++) PUT(P.l0[i], v->l0[i + D * p]);
+    if (M.tpic) for (int i = 0; i < DD; i++) PUT(P.gv[i], v->gradv[i + DD * p]);
+#undef PUT
+    return EP_OK;
+}
+
 extern "C" int ep_upload_particles(ep_ctx *c, const ep_particles_view *v) {
     if (!c || !v) return fail(EP_E_ARG, "ep_upload_particles: null argument");
     if (!c->grid_ready) return fail(EP_E_STATE, "upload the grid first");
@@ -310,67 +393,10 @@ extern "C" int ep_upload_particles(ep_ctx *c, const ep_particles_view *v) {
     c->perm.resize(n);
     std::iota(c->perm.begin(), c->perm.end(), 0L);
     std::stable_sort(c->perm.begin(), c->perm.end(), [&](long a, long b) { return key[a] < key[b]; });
-    // one device allocation per particle field
-    for (void *a : c->part_allocs) hipFree(a);
-    c->part_allocs.clear();
-    c->nmp = n;
-    PState<double> &P = c->ps;
-    P = PState<double>{};
-    bool oom = false;
-    auto take = [&]() -> double * {
-        void *a = nullptr;
-        if (hipMalloc(&a, sizeof(double) * n) != hipSuccess) { oom = true; return nullptr; }
-        c->part_allocs.push_back(a);
-        return static_cast<double *>(a);
-    };
-    for (int k = 0; k < 2; k++) for (int i = 0; i < D; i++) P.x[k][i] = take();
-    for (int i = 0; i < D; i++) P.v[i] = take();
-    for (int i = 0; i < DD; i++) P.F[i] = take();
-    for (int i = 0; i < (D == 2 ? 3 : 6); i++) P.b[i] = take();
-    for (int i = 0; i < NS; i++) P.s[i] = take();
-    P.epII = take(); P.dlam = take(); P.J = take(); P.rho = take();
-    P.Om0 = take(); P.c0 = take(); P.cr = take(); P.phi = take();
-    if (c->cfg.basis == EP_GIMPM) for (int i = 0; i < D; i++) P.l0[i] = take();
-    if (M.tpic) for (int i = 0; i < DD; i++) P.gv[i] = take();
-    if (M.locking) { for (int i = 0; i < DD; i++) P.dF[i] = take(); P.dJ = take(); }
-    if (oom) return fail(EP_E_HIP, "hipMalloc of particle fields failed");
-    P.n = n;
-    // host staging in device order
-    std::vector<double> st(n);
-    auto put = [&](double *dst, auto getter) -> int {
-        for (long q = 0; q < n; q++) st[q] = getter(c->perm[q]);
-        hipError_t e = hipMemcpy(dst, st.data(), sizeof(double) * n, hipMemcpyHostToDevice);
-        return e == hipSuccess ? EP_OK : fail(EP_E_HIP, "hipMemcpy particle upload failed");
-    };
-    int r;
-#define PUT(dst, expr) do { if ((r = put(dst, [&](long p) { return (expr); })) != EP_OK) return r; } while (0)
-    for (int i = 0; i < D; i++) { PUT(P.x[0][i], v->x[i + D * p]); PUT(P.x[1][i], v->x[i + D * p]); }
-    for (int i = 0; i < D; i++) PUT(P.v[i], v->v[i + D * p]);
-    for (int i = 0; i < DD; i++) PUT(P.F[i], v->F[i + DD * p]);
-    if (M.finite) {
-        // symmetric part of bᵢⱼ: 2D xx,yy,xy; 3D xx,yy,zz,yz,xz,xy
-        static const int i2[3][2] = {{0, 0}, {1, 1}, {0, 1}};
-        static const int i3[6][2] = {{0, 0}, {1, 1}, {2, 2}, {1, 2}, {0, 2}, {0, 1}};
-        for (int k = 0; k < (D == 2 ? 3 : 6); k++) {
-            const int a = D == 2 ? i2[k][0] : i3[k][0], bb = D == 2 ? i2[k][1] : i3[k][1];
-            PUT(P.b[k], 0.5 * (v->b[a + D * bb + DD * p] + v->b[bb + D * a + DD * p]));
-        }
-        for (int k = 0; k < NS; k++) PUT(P.s[k], v->tau[k + NS * p]);
-    } else {
-        for (int k = 0; k < (D == 2 ? 3 : 6); k++) PUT(P.b[k], (k < D) ? 1.0 : 0.0);
-        for (int k = 0; k < NS; k++) PUT(P.s[k], v->sig[k + NS * p]);
+    {
+        const int r = c->fp32 ? upload_fields<float>(c, v, c->psf) : upload_fields<double>(c, v, c->ps);
+        if (r != EP_OK) return r;
     }
-    PUT(P.epII, v->epII[0 + 2 * p]);
-    PUT(P.dlam, v->dlam ? v->dlam[p] : 0.0);
-    PUT(P.J, v->J[p]);
-    PUT(P.rho, M.finite ? v->rho0[p] : v->rho[p]);
-    PUT(P.Om0, v->Om0[p]);
-    PUT(P.c0, v->c0[p]);
-    PUT(P.cr, v->cr[p]);
-    PUT(P.phi, v->phi[p]);
-    if (c->cfg.basis == EP_GIMPM) for (int i = 0; i < D; i++) PUT(P.l0[i], v->l0[i + D * p]);
-    if (M.tpic) for (int i = 0; i < DD; i++) PUT(P.gv[i], v->gradv[i + DD * p]);
-#undef PUT
     c->n0_h.assign(n, 0.1);
     if (v->n0) for (long p = 0; p < n; p++) c->n0_h[p] = v->n0[p];
     c->rho0_h.assign(v->rho0, v->rho0 + n);
@@ -406,20 +432,22 @@ extern "C" int ep_set_time(ep_ctx *c, double t) {
     return EP_OK;
 }
 
-extern "C" int ep_download_particles(ep_ctx *c, ep_particles_view *v) {
-    if (!c || !v) return fail(EP_E_ARG, "ep_download_particles: null argument");
-    if (!c->part_ready) return fail(EP_E_STATE, "no particles uploaded");
-    HIPCHK(hipStreamSynchronize(c->stream));
+template <typename T>
+static int download_fields(ep_ctx *c, ep_particles_view *v, const PState<T> &P, int par) {
     const int D = c->dim, NS = c->nstr, DD = D * D;
     const long n = c->nmp;
     const Mat &M = c->mat;
-    const PState<double> &P = c->ps;
-    Ctrl h{};
-    HIPCHK(hipMemcpy(&h, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
-    const int par = h.parity;
     std::vector<double> st(n);
-    auto get = [&](const double *src) -> int {
-        hipError_t e = hipMemcpy(st.data(), src, sizeof(double) * n, hipMemcpyDeviceToHost);
+    std::vector<T> stt(n);
+    auto get = [&](const auto *src) -> int {
+        using E = std::remove_cv_t<std::remove_pointer_t<decltype(src)>>;
+        hipError_t e;
+        if constexpr (std::is_same<E, double>::value) {
+            e = hipMemcpy(st.data(), src, sizeof(double) * n, hipMemcpyDeviceToHost);
+        } else {
+            e = hipMemcpy(stt.data(), src, sizeof(E) * n, hipMemcpyDeviceToHost);
+            for (long q = 0; q < n; q++) st[q] = double(stt[q]);
+        }
         return e == hipSuccess ? EP_OK : fail(EP_E_HIP, "hipMemcpy particle download failed");
     };
     int r;
@@ -440,7 +468,8 @@ extern "C" int ep_download_particles(ep_ctx *c, ep_particles_view *v) {
         static const int i3[6][2] = {{0, 0}, {1, 1}, {2, 2}, {1, 2}, {0, 2}, {0, 1}};
         for (int k = 0; k < (D == 2 ? 3 : 6); k++) {
             const int a = D == 2 ? i2[k][0] : i3[k][0], bb = D == 2 ? i2[k][1] : i3[k][1];
-            GET(P.b[k], (v->b[a + D * bb + DD * p] = val, v->b[bb + D * a + DD * p] = val));
+            const double off = (c->fp32 && a == bb) ? 1.0 : 0.0;
+            GET(P.b[k], (v->b[a + D * bb + DD * p] = val + off, v->b[bb + D * a + DD * p] = val + off));
         }
     }
     tau.assign(size_t(NS) * n, 0.0);
@@ -470,9 +499,25 @@ extern "C" int ep_download_particles(ep_ctx *c, ep_particles_view *v) {
     if (v->epII) GET(P.epII, (v->epII[0 + 2 * p] = val, v->epII[1 + 2 * p] = val));
     if (v->dlam) GET(P.dlam, v->dlam[p] = val);
     if (v->gradv && M.tpic) for (int i = 0; i < DD; i++) GET(P.gv[i], v->gradv[i + DD * p] = val);
-    if (v->dJ && M.locking) GET(P.dJ, v->dJ[p] = val);
     if (v->l && c->cfg.basis == EP_GIMPM) for (int i = 0; i < D; i++) GET(P.l0[i], v->l[i + D * p] = val);
 #undef GET
+    return EP_OK;
+}
+
+extern "C" int ep_download_particles(ep_ctx *c, ep_particles_view *v) {
+    if (!c || !v) return fail(EP_E_ARG, "ep_download_particles: null argument");
+    if (!c->part_ready) return fail(EP_E_STATE, "no particles uploaded");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int D = c->dim, NS = c->nstr, DD = D * D;
+    const long n = c->nmp;
+    const Mat &M = c->mat;
+    Ctrl h{};
+    HIPCHK(hipMemcpy(&h, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+    {
+        const int r = c->fp32 ? download_fields<float>(c, v, c->psf, h.parity)
+                              : download_fields<double>(c, v, c->ps, h.parity);
+        if (r != EP_OK) return r;
+    }
     if (v->vmax)
         for (int i = 0; i < D; i++) std::memcpy(&v->vmax[i], &h.vmax[i], 8);
     return EP_OK;
@@ -524,8 +569,8 @@ static void timed(ep_ctx *c, int kid, L &&launch) {
     c->stats.launches++;
 }
 
-template <int D, int B>
-static void launch_step(ep_ctx *c) {
+template <int D, int B, typename T>
+static void launch_step(ep_ctx *c, const PState<T> &P) {
     const int TB = EP_BLOCK;
     const long n = c->nmp, nno = c->gs.nno;
     const dim3 gp((n + TB - 1) / TB), gg((nno + TB - 1) / TB);
@@ -533,30 +578,36 @@ static void launch_step(ep_ctx *c) {
     Ctrl *ct = c->ctrl;
     const Geom g = c->geom;
     const GState G = c->gs;
-    const PState<double> P = c->ps;
     const Mat M = c->mat;
     timed(c, K_BEGIN, [&] { EP_LAUNCH(k_begin<D>, 1, 64, s, ct, g, M); });
     timed(c, K_ZERO, [&] { EP_LAUNCH(k_zero_grid<D>, gg, TB, s, ct, G); });
-    timed(c, K_P2G, [&] { EP_LAUNCH((k_p2g<D, B, double>), gp, TB, s, ct, g, P, G, M); });
+    timed(c, K_P2G, [&] { EP_LAUNCH((k_p2g<D, B, T>), gp, TB, s, ct, g, P, G, M); });
     timed(c, K_EULER, [&] { EP_LAUNCH(k_euler<D>, gg, TB, s, ct, G, M); });
-    timed(c, K_G2P, [&] { EP_LAUNCH((k_g2p<D, B, double>), gp, TB, s, ct, g, P, G, M); });
+    timed(c, K_G2P, [&] { EP_LAUNCH((k_g2p<D, B, T>), gp, TB, s, ct, g, P, G, M); });
     if (M.musl) timed(c, K_AUGV, [&] { EP_LAUNCH(k_augv<D>, gg, TB, s, ct, G); });
-    timed(c, K_DEFORM, [&] { EP_LAUNCH((k_deform<D, B, double>), gp, TB, s, ct, g, P, G, M); });
-    if (M.locking) timed(c, K_FBAR, [&] { EP_LAUNCH((k_fbar<D, B, double>), gp, TB, s, ct, g, P, G, M); });
-    timed(c, K_END, [&] { EP_LAUNCH(k_end, 1, EP_BLOCK, s, ct); });
+    if (M.locking) timed(c, K_DEFORM, [&] { EP_LAUNCH((k_deform<D, B, T, true>), gp, TB, s, ct, g, P, G, M); });
+    else timed(c, K_DEFORM, [&] { EP_LAUNCH((k_deform<D, B, T, false>), gp, TB, s, ct, g, P, G, M); });
+    if (M.locking) timed(c, K_FBAR, [&] { EP_LAUNCH((k_fbar<D, B, T>), gp, TB, s, ct, g, P, G, M); });
+    timed(c, K_END, [&] { EP_LAUNCH(k_end, 1, 4 * EP_BLOCK, s, ct); });
+}
+
+template <typename T>
+static void dispatch_step_t(ep_ctx *c, const PState<T> &P) {
+    const int D = c->dim, B = c->cfg.basis;
+    if (D == 2) {
+        if (B == EP_BSMPM) launch_step<2, 0, T>(c, P);
+        else if (B == EP_GIMPM) launch_step<2, 1, T>(c, P);
+        else launch_step<2, 2, T>(c, P);
+    } else {
+        if (B == EP_BSMPM) launch_step<3, 0, T>(c, P);
+        else if (B == EP_GIMPM) launch_step<3, 1, T>(c, P);
+        else launch_step<3, 2, T>(c, P);
+    }
 }
 
 static void dispatch_step(ep_ctx *c) {
-    const int D = c->dim, B = c->cfg.basis;
-    if (D == 2) {
-        if (B == EP_BSMPM) launch_step<2, 0>(c);
-        else if (B == EP_GIMPM) launch_step<2, 1>(c);
-        else launch_step<2, 2>(c);
-    } else {
-        if (B == EP_BSMPM) launch_step<3, 0>(c);
-        else if (B == EP_GIMPM) launch_step<3, 1>(c);
-        else launch_step<3, 2>(c);
-    }
+    if (c->fp32) dispatch_step_t<float>(c, c->psf);
+    else dispatch_step_t<double>(c, c->ps);
 }
 
 extern "C" int ep_step(ep_ctx *c, int32_t nsteps, const ep_step_params *par, ep_step_result *res) {

@@ -11,6 +11,7 @@
 // alternate between steps, selected by ctrl->parity on the device).
 #pragma once
 #include "ep_math.h"
+#include <type_traits>
 
 namespace ep {
 
@@ -20,6 +21,7 @@ struct Geom {
     int nno[3];
     int nel[3];
     double h[3];
+    double ih[3];              // 1/h
     double x0[3];
     const double *X[3];        // per-axis node coordinates
     const int8_t *T[3];        // per-axis B-spline node type (1..4, 0 invalid)
@@ -28,10 +30,11 @@ struct Geom {
 
 template <typename T>
 struct PState {
-    T *x[2][3];                // two position buffers, ctrl->parity picks
+    double *x[2][3];           // positions (FP64 in every path), two buffers, ctrl->parity picks
     T *v[3];
     T *F[9];                   // F[i + d*j]
-    T *b[6];                   // symmetric: 2D xx,yy,xy; 3D xx,yy,zz,yz,xz,xy
+    T *b[6];                   // symmetric b (FP64 path) or B = b - I (FP32 path):
+                               // 2D xx,yy,xy; 3D xx,yy,zz,yz,xz,xy
     T *s[6];                   // τ (finite) or σ (infinitesimal), Voigt
     T *epII, *dlam, *J;
     T *rho;                    // ρ₀ (finite: ρ = ρ₀/J) or the constant ρ (infinitesimal)
@@ -85,36 +88,37 @@ struct W {
 // shpfun: p2e2n + basis (tplgy.jl:43-96, bsmpm.jl:95-135, gimpm.jl:31-71,
 // smpm.jl:27-66).  Returns false on a bsmpm "Invalid position".
 template <int D, int B, typename T>
-__device__ __forceinline__ bool weights(const Geom &g, const T *xp, const T *lp, W<D, T> &w) {
+__device__ __forceinline__ bool weights(const Geom &g, const double *xp, const T *lp, W<D, T> &w) {
     bool good = true;
     w.ok = 0u;
 #pragma unroll
     for (int i = 0; i < D; i++) {
-        const double a = double(xp[i]) - g.x0[i];
-        double e = exact_fld(a, g.h[i]);
+        const double a = xp[i] - g.x0[i];
+        double e = exact_fld(a, g.h[i], g.ih[i]);
         const double hi = double(g.nel[i] - 1);
         e = e < 0.0 ? 0.0 : (e > hi ? hi : e);
         const int base = int(e) - 1;
         w.base[i] = base;
-        const T h = T(g.h[i]);
+        const T h = T(g.h[i]), ih = T(g.ih[i]);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const int n = base + k;
             T f = T(0), df = T(0);
             if (n >= 0 && n < g.nno[i]) {
                 w.ok |= 1u << (4 * i + k);
-                const T xn = T(g.X[i][n]);
+                // x_p - x_n is formed in FP64 before the weight arithmetic
+                const double dxd = xp[i] - g.X[i][n];
                 if (B == 0) {
                     const int ty = g.T[i][n];
                     if (ty == 0) good = false;
                     T ff, dd;
-                    bspline<T>((xp[i] - xn) / h, ty, ff, dd);
+                    bspline<T>(T(dxd * g.ih[i]), ty, ff, dd);
                     f = ff;
-                    df = dd / h;
+                    df = dd * ih;
                 } else if (B == 1) {
-                    gimp<T>(xp[i] - xn, h, lp[i], f, df);
+                    gimp<T>(T(dxd), h, lp[i], f, df);
                 } else {
-                    linear<T>(xp[i] - xn, h, f, df);
+                    linear<T>(T(dxd), h, f, df);
                 }
             }
             w.w[i][k] = f;
@@ -391,11 +395,11 @@ __device__ __forceinline__ unsigned long long block_max_bits(unsigned long long 
     return b;
 }
 
-template <int D, typename T>
-__device__ __forceinline__ void stencil_base(const Geom &g, const T *xp, int *base) {
+template <int D>
+__device__ __forceinline__ void stencil_base(const Geom &g, const double *xp, int *base) {
 #pragma unroll
     for (int i = 0; i < D; i++) {
-        double e = exact_fld(double(xp[i]) - g.x0[i], g.h[i]);
+        double e = exact_fld(xp[i] - g.x0[i], g.h[i], g.ih[i]);
         const double hi = double(g.nel[i] - 1);
         e = e < 0.0 ? 0.0 : (e > hi ? hi : e);
         base[i] = int(e) - 1;
@@ -440,14 +444,20 @@ __global__ void k_begin(Ctrl *c, Geom g, Mat M) {
 __global__ void k_end(Ctrl *c) {
     __shared__ unsigned long long sred[16];
     if (!c->active) return;
-    for (int d = 0; d < 3; d++) {
-        unsigned long long b = 0ull;
-        for (int k = threadIdx.x; k < c->nblk; k += blockDim.x) {
-            const unsigned long long o = c->vmax_part[d * c->nblk + k];
-            b = o > b ? o : b;
+    const int nblk = c->nblk;
+    const unsigned long long *part = c->vmax_part;
+    unsigned long long b[3] = {0ull, 0ull, 0ull};
+    for (int k = threadIdx.x; k < nblk; k += blockDim.x) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const unsigned long long o = part[d * nblk + k];
+            b[d] = o > b[d] ? o : b[d];
         }
-        b = block_max_bits(b, sred);
-        if (threadIdx.x == 0) c->vmax[d] = b;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const unsigned long long r = block_max_bits(b[d], sred);
+        if (threadIdx.x == 0) c->vmax[d] = r;
     }
     if (threadIdx.x != 0) return;
     c->t = c->t + c->dt;
@@ -479,7 +489,8 @@ __global__ void __launch_bounds__(256) k_p2g(const Ctrl *c, Geom g, PState<T> P,
     const bool live = p < P.n;
     const long q = live ? p : 0;
     const int par = c->parity;
-    T xp[D], lp[D];
+    double xp[D];
+    T lp[D];
 #pragma unroll
     for (int i = 0; i < D; i++) { xp[i] = P.x[par][i][q]; lp[i] = B == 1 ? P.l0[i][q] : T(0); }
     W<D, T> w;
@@ -517,8 +528,8 @@ __global__ void __launch_bounds__(256) k_p2g(const Ctrl *c, Geom g, PState<T> P,
         if (D == 2) {
             if (tpic) {
                 // Δnp = x_n - x_p (misc.jl:1-14)
-                const T dx = T(g.X[0][ix[0]]) - xp[0];
-                const T dy = T(g.X[1][ix[1]]) - xp[1];
+                const T dx = T(g.X[0][ix[0]] - xp[0]);
+                const T dy = T(g.X[1][ix[1]] - xp[1]);
                 sink(1, no, li, double(N * ms * (v[0] + gv[0][0] * dx + gv[0][1] * dy)));
                 sink(2, no, li, double(N * ms * (v[1] + gv[1][0] * dx + gv[1][1] * dy)));
             } else {
@@ -529,9 +540,9 @@ __global__ void __launch_bounds__(256) k_p2g(const Ctrl *c, Geom g, PState<T> P,
             sink(4, no, li, double(-(Om * (dN[0] * s[2] + dN[1] * s[1]) - N * (ms * gz))));
         } else {
             if (tpic) {
-                const T dx = T(g.X[0][ix[0]]) - xp[0];
-                const T dy = T(g.X[1][ix[1]]) - xp[1];
-                const T dz = T(g.X[2][ix[2]]) - xp[2];
+                const T dx = T(g.X[0][ix[0]] - xp[0]);
+                const T dy = T(g.X[1][ix[1]] - xp[1]);
+                const T dz = T(g.X[2][ix[2]] - xp[2]);
                 sink(1, no, li, double(N * ms * (v[0] + gv[0][0] * dx + gv[0][1] * dy + gv[0][2] * dz)));
                 sink(2, no, li, double(N * ms * (v[1] + gv[1][0] * dx + gv[1][1] * dy + gv[1][2] * dz)));
                 sink(3, no, li, double(N * ms * (v[2] + gv[2][0] * dx + gv[2][1] * dy + gv[2][2] * dz)));
@@ -601,7 +612,8 @@ __global__ void __launch_bounds__(256) k_g2p(Ctrl *c, Geom g, PState<T> P, GStat
     const bool live = p < P.n;
     const long q = live ? p : 0;
     const int par = c->parity;
-    T xp[D], lp[D];
+    double xp[D];
+    T lp[D];
 #pragma unroll
     for (int i = 0; i < D; i++) { xp[i] = P.x[par][i][q]; lp[i] = B == 1 ? P.l0[i][q] : T(0); }
     W<D, T> w;
@@ -640,7 +652,7 @@ __global__ void __launch_bounds__(256) k_g2p(Ctrl *c, Geom g, PState<T> P, GStat
 #pragma unroll
     for (int d = 0; d < D; d++) {
         vn[d] = C * (P.v[d][q] + dt * aflip[d]) + (T(1) - C) * vpic[d];
-        const T xn = xp[d] + dt * vpic[d];
+        const double xn = xp[d] + double(dt * vpic[d]);
         if (live) {
             P.v[d][p] = vn[d];
             P.x[par ^ 1][d][p] = xn;
@@ -686,21 +698,37 @@ __global__ void k_augv(const Ctrl *c, GState G) {
     }
 }
 
-// elastic predictor + plastic corrector on one particle (elast.jl:45-80,
-// plast.jl:49-65, DP.jl:33-99, J2.jl:21-75).  dF is the (F-bar scaled) ΔF.
+// The FP32 path carries the kinematics as deviations from the identity so
+// that small strains keep their relative precision: K = ΔF - I instead of
+// ΔF, δJ = ΔJ - 1, B = b - I, ε = ½ log1p(eig B), b_new - I = expm1(2ε).
+template <typename T> constexpr bool kDev = std::is_same<T, float>::value;
+
+// det(I + A) - 1 expanded so no 1 + small is rounded first
 template <int D, typename T>
-__device__ __forceinline__ void constitutive(Ctrl *c, const PState<T> &P, const Mat &M, long p, const T (&dF)[D][D]) {
-    constexpr int NS = D == 2 ? 3 : 6;
-    T s[NS];
-    if (M.finite) {
-        // b <- ΔF b ΔF'  (elast.jl:49)
-        T b[D][D], t1[D][D], bn[D][D];
-        if (D == 2) {
-            b[0][0] = P.b[0][p]; b[1][1] = P.b[1][p]; b[0][1] = b[1][0] = P.b[2][p];
-        } else {
-            b[0][0] = P.b[0][p]; b[1][1] = P.b[1][p]; b[2][2] = P.b[2][p];
-            b[1][2] = b[2][1] = P.b[3][p]; b[0][2] = b[2][0] = P.b[4][p]; b[0][1] = b[1][0] = P.b[5][p];
-        }
+__device__ __forceinline__ T det1m(const T (&A)[D][D]) {
+    if (D == 2) return A[0][0] + A[1][1] + (A[0][0] * A[1][1] - A[0][1] * A[1][0]);
+    const T tr = A[0][0] + A[1][1] + A[D - 1][D - 1];
+    const T m2 = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) + (A[0][0] * A[D - 1][D - 1] - A[0][D - 1] * A[D - 1][0]) +
+                 (A[1][1] * A[D - 1][D - 1] - A[1][D - 1] * A[D - 1][1]);
+    const T d3 = A[0][0] * (A[1][1] * A[D - 1][D - 1] - A[1][D - 1] * A[D - 1][1]) -
+                 A[0][1] * (A[1][0] * A[D - 1][D - 1] - A[1][D - 1] * A[D - 1][0]) +
+                 A[0][D - 1] * (A[1][0] * A[D - 1][1] - A[1][1] * A[D - 1][0]);
+    return tr + m2 + d3;
+}
+
+// b <- ΔF b ΔF' (elast.jl:49) on the stored symmetric b
+template <int D, typename T>
+__device__ __forceinline__ void push_b(const PState<T> &P, long p, const T (&dF)[D][D], T (&bn)[D][D]) {
+    T b[D][D], t1[D][D];
+    if (D == 2) {
+        b[0][0] = P.b[0][p]; b[1][1] = P.b[1][p]; b[0][1] = b[1][0] = P.b[2][p];
+    } else {
+        b[0][0] = P.b[0][p]; b[1][1] = P.b[1][p]; b[D - 1][D - 1] = P.b[D - 1][p];
+        b[1][D - 1] = b[D - 1][1] = P.b[3][p]; b[0][D - 1] = b[D - 1][0] = P.b[4][p]; b[0][1] = b[1][0] = P.b[5][p];
+    }
+    if constexpr (kDev<T>) {
+        // B' = K + K' + B + K B + B K' + K K' + K B K'   (K = ΔF - I)
+        T KB[D][D];
 #pragma unroll
         for (int i = 0; i < D; i++)
 #pragma unroll
@@ -708,50 +736,176 @@ __device__ __forceinline__ void constitutive(Ctrl *c, const PState<T> &P, const 
                 T a = dF[i][0] * b[0][j];
 #pragma unroll
                 for (int k = 1; k < D; k++) a += dF[i][k] * b[k][j];
-                t1[i][j] = a;
+                KB[i][j] = a + b[i][j] + dF[i][j];      // K B + B + K
             }
 #pragma unroll
         for (int i = 0; i < D; i++)
 #pragma unroll
             for (int j = i; j < D; j++) {
-                T a = t1[i][0] * dF[j][0];
+                T a = KB[i][0] * dF[j][0];
 #pragma unroll
-                for (int k = 1; k < D; k++) a += t1[i][k] * dF[j][k];
-                bn[i][j] = a;
-                bn[j][i] = a;
+                for (int k = 1; k < D; k++) a += KB[i][k] * dF[j][k];   // (K B + B + K) K'
+                const T v = KB[i][j] + dF[j][i] + a;
+                bn[i][j] = v;
+                bn[j][i] = v;
             }
-        // ε = ½ log b (elast.jl:51-52), τ = Del voigt(ε, 2) (elast.jl:54)
-        T a[D][D], V[D][D];
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+            T a = dF[i][0] * b[0][j];
+#pragma unroll
+            for (int k = 1; k < D; k++) a += dF[i][k] * b[k][j];
+            t1[i][j] = a;
+        }
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+        for (int j = i; j < D; j++) {
+            T a = t1[i][0] * dF[j][0];
+#pragma unroll
+            for (int k = 1; k < D; k++) a += t1[i][k] * dF[j][k];
+            bn[i][j] = a;
+            bn[j][i] = a;
+        }
+}
+
+template <int D, typename T>
+__device__ __forceinline__ void store_sym(T *const *dst, long p, const T (&m)[D][D]) {
+    if (D == 2) { dst[0][p] = m[0][0]; dst[1][p] = m[1][1]; dst[2][p] = m[0][1]; }
+    else {
+        dst[0][p] = m[0][0]; dst[1][p] = m[1][1]; dst[2][p] = m[D - 1][D - 1];
+        dst[3][p] = m[1][D - 1]; dst[4][p] = m[0][D - 1]; dst[5][p] = m[0][1];
+    }
+}
+
+// M = V diag(f) V'
+template <int D, typename T>
+__device__ __forceinline__ void vdvt(const T (&V)[D][D], const T *f, T (&m)[D][D]) {
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+        for (int j = i; j < D; j++) {
+            T acc = T(0);
+#pragma unroll
+            for (int k = 0; k < D; k++) acc += (V[i][k] * f[k]) * V[j][k];
+            m[i][j] = acc;
+            m[j][i] = acc;
+        }
+}
+
+template <int D, typename T>
+__device__ __forceinline__ void sym_to_voigt(const T (&m)[D][D], T *s) {
+    if (D == 2) { s[0] = m[0][0]; s[1] = m[1][1]; s[2] = m[0][1]; }
+    else { s[0] = m[0][0]; s[1] = m[1][1]; s[2] = m[D - 1][D - 1]; s[3] = m[1][D - 1]; s[4] = m[0][D - 1]; s[5] = m[0][1]; }
+}
+
+// Elastic predictor + plastic corrector on one particle (elast.jl:45-80,
+// plast.jl:49-65, DP.jl:33-99, J2.jl:21-75); dF is the (F-bar scaled) ΔF,
+// or ΔF - I on the FP32 path.
+//
+// Finite strain with Drucker-Prager (or no plasticity) runs in the principal
+// frame of the trial b: ε = ½ log b, τ = Del·ε and the return map
+// (isotropic Del, ψ = 0) all keep b's eigenvectors, so one eigen-
+// decomposition gives ε and τ (elast.jl:51-54), the DP return acts on the
+// principal stresses (its invariants are frame-independent, DP.jl:1-12) and
+// the recomputed b = exp(2 tensor(Del\τ, ½)) (DP.jl:63-66) is
+// V diag(exp 2ε_new) V' with the same V.  The reference's second eigen-
+// decomposition of ε_new yields the same matrix up to rounding.  The J2
+// cutting plane weights shear with Del's G against the 2·s² of J₂, which is
+// not coaxial, so J2 keeps the reference's Voigt-space iteration.
+template <int D, typename T>
+__device__ __forceinline__ void constitutive(Ctrl *c, const PState<T> &P, const Mat &M, long p, const T (&dF)[D][D]) {
+    constexpr int NS = D == 2 ? 3 : 6;
+    T s[NS];
+    bool bad = false;
+    if (M.finite) {
+        T bn[D][D], a[D][D], V[D][D];
+        push_b<D, T>(P, p, dF, bn);
 #pragma unroll
         for (int i = 0; i < D; i++)
 #pragma unroll
             for (int j = 0; j < D; j++) a[i][j] = bn[i][j];
         sym_eig<D, T>(a, V);
-        T fl[D];
+        if constexpr (std::is_same<T, float>::value) {
+            // FP64 island of the FP32 path: near-repeated eigenvalues of
+            // B = b - I are decomposed again in FP64.
+            T mx = T(0), gap = T(1e30f);
 #pragma unroll
-        for (int k = 0; k < D; k++) fl[k] = log(a[k][k]);
-        T e[D][D];
+            for (int i = 0; i < D; i++) {
+                mx = fmax(mx, fabs(a[i][i]));
 #pragma unroll
-        for (int i = 0; i < D; i++)
-#pragma unroll
-            for (int j = 0; j < D; j++) {
-                T acc = T(0);
-#pragma unroll
-                for (int k = 0; k < D; k++) acc += (V[i][k] * fl[k]) * V[j][k];
-                e[i][j] = T(0.5) * acc;
+                for (int j = i + 1; j < D; j++) gap = fmin(gap, fabs(a[i][i] - a[j][j]));
             }
-        T ev[NS];
-        if (D == 2) { ev[0] = e[0][0]; ev[1] = e[1][1]; ev[2] = T(2) * e[0][1]; }
-        else { ev[0] = e[0][0]; ev[1] = e[1][1]; ev[2] = e[2][2]; ev[3] = T(2) * e[1][2]; ev[4] = T(2) * e[0][2]; ev[5] = T(2) * e[0][1]; }
+            if (gap < T(1e-4f) * mx) {
+                double ad[D][D], Vd[D][D];
+#pragma unroll
+                for (int i = 0; i < D; i++)
+#pragma unroll
+                    for (int j = 0; j < D; j++) ad[i][j] = double(bn[i][j]);
+                sym_eig<D, double>(ad, Vd);
+#pragma unroll
+                for (int i = 0; i < D; i++)
+#pragma unroll
+                    for (int j = 0; j < D; j++) { a[i][j] = T(ad[i][j]); V[i][j] = T(Vd[i][j]); }
+            }
+        }
+        T ep[D], sp[NS];
+#pragma unroll
+        for (int k = 0; k < D; k++) ep[k] = kDev<T> ? T(0.5) * log1p(a[k][k]) : T(0.5) * log(a[k][k]);
+        // principal Kirchhoff stress: Del on [ε1, ε2(, ε3), 0...]
 #pragma unroll
         for (int i = 0; i < NS; i++) {
             T acc = T(0);
 #pragma unroll
-            for (int j = 0; j < NS; j++) acc += T(M.Del[i + NS * j]) * ev[j];
-            s[i] = acc;
+            for (int j = 0; j < D; j++) acc += T(M.Del[i + NS * j]) * ep[j];
+            sp[i] = acc;
         }
-        if (D == 2) { P.b[0][p] = bn[0][0]; P.b[1][p] = bn[1][1]; P.b[2][p] = bn[0][1]; }
-        else { P.b[0][p] = bn[0][0]; P.b[1][p] = bn[1][1]; P.b[2][p] = bn[2][2]; P.b[3][p] = bn[1][2]; P.b[4][p] = bn[0][2]; P.b[5][p] = bn[0][1]; }
+        const bool plast = c->with_plast && M.plast != 0;
+        if (plast && M.plast == 1) {
+            T e2 = P.epII[p], e1 = e2, dl;
+            dp_return<NS, T>(sp, P.c0[p], P.cr[p], P.phi[p], T(M.Hp), T(M.Kc), T(M.Gc), e2, e1, dl);
+            P.dlam[p] = dl;
+            P.epII[p] = e1;
+            T y[NS];
+            lu_solve<NS, T>(M.lu, sp, y);
+            T fe[D];
+#pragma unroll
+            for (int k = 0; k < D; k++) fe[k] = kDev<T> ? expm1(T(2) * y[k]) : exp(T(2) * y[k]);
+            vdvt<D, T>(V, fe, bn);
+        }
+        T tp[D];
+#pragma unroll
+        for (int k = 0; k < D; k++) tp[k] = sp[k];
+        T tm[D][D];
+        vdvt<D, T>(V, tp, tm);
+        sym_to_voigt<D, T>(tm, s);
+        if (plast && M.plast == 2) {
+            T e2 = P.epII[p], e1 = e2;
+            const int it = j2_return<NS, T>(s, P.c0[p], P.cr[p], T(M.Hp), M.Del, e2, e1);
+            if (it != 0) {
+                P.epII[p] = e1;
+                if (it < 0) atomicAdd(&c->j2cap, 1);
+                // ε = tensor(Del \ τ, ½); b = exp(2ε) (J2.jl:42-45)
+                T y[NS];
+                lu_solve<NS, T>(M.lu, s, y);
+                T e[D][D], W2[D][D];
+                if (D == 2) { e[0][0] = y[0]; e[1][1] = y[1]; e[0][1] = e[1][0] = T(0.5) * y[2]; }
+                else {
+                    e[0][0] = y[0]; e[1][1] = y[1]; e[D - 1][D - 1] = y[D - 1];
+                    e[1][D - 1] = e[D - 1][1] = T(0.5) * y[3]; e[0][D - 1] = e[D - 1][0] = T(0.5) * y[4];
+                    e[0][1] = e[1][0] = T(0.5) * y[5];
+                }
+                sym_eig<D, T>(e, W2);
+                T fe[D];
+#pragma unroll
+                for (int k = 0; k < D; k++) fe[k] = kDev<T> ? expm1(T(2) * e[k][k]) : exp(T(2) * e[k][k]);
+                vdvt<D, T>(W2, fe, bn);
+            }
+        }
+        store_sym<D, T>(P.b, p, bn);
     } else {
         // infinitesimal_elast (elast.jl:69-80); the Jaumann term σω'+σ'ω is
         // identically zero for symmetric σ
@@ -760,9 +914,11 @@ __device__ __forceinline__ void constitutive(Ctrl *c, const PState<T> &P, const 
 #pragma unroll
         for (int i = 0; i < D; i++)
 #pragma unroll
-            for (int j = 0; j < D; j++) e[i][j] = T(0.5) * (dF[i][j] + dF[j][i]) - (i == j ? T(1) : T(0));
+            for (int j = 0; j < D; j++)
+                e[i][j] = kDev<T> ? T(0.5) * (dF[i][j] + dF[j][i])
+                                  : T(0.5) * (dF[i][j] + dF[j][i]) - (i == j ? T(1) : T(0));
         if (D == 2) { ev[0] = e[0][0]; ev[1] = e[1][1]; ev[2] = T(2) * e[0][1]; }
-        else { ev[0] = e[0][0]; ev[1] = e[1][1]; ev[2] = e[2][2]; ev[3] = T(2) * e[1][2]; ev[4] = T(2) * e[0][2]; ev[5] = T(2) * e[0][1]; }
+        else { ev[0] = e[0][0]; ev[1] = e[1][1]; ev[2] = e[D - 1][D - 1]; ev[3] = T(2) * e[1][D - 1]; ev[4] = T(2) * e[0][D - 1]; ev[5] = T(2) * e[0][1]; }
 #pragma unroll
         for (int i = 0; i < NS; i++) {
             T acc = T(0);
@@ -770,53 +926,20 @@ __device__ __forceinline__ void constitutive(Ctrl *c, const PState<T> &P, const 
             for (int j = 0; j < NS; j++) acc += T(M.Del[i + NS * j]) * ev[j];
             s[i] = P.s[i][p] + acc;
         }
-    }
-    bool bnew = false;
-    if (c->with_plast && M.plast != 0) {
-        T e2 = P.epII[p], e1 = e2;
-        if (M.plast == 1) {
-            T dl;
+        if (c->with_plast && M.plast == 1) {
+            T e2 = P.epII[p], e1 = e2, dl;
             dp_return<NS, T>(s, P.c0[p], P.cr[p], P.phi[p], T(M.Hp), T(M.Kc), T(M.Gc), e2, e1, dl);
             P.dlam[p] = dl;
             P.epII[p] = e1;
-            bnew = M.finite;
-        } else {
+        } else if (c->with_plast && M.plast == 2) {
+            T e2 = P.epII[p], e1 = e2;
             const int it = j2_return<NS, T>(s, P.c0[p], P.cr[p], T(M.Hp), M.Del, e2, e1);
             if (it != 0) {
                 P.epII[p] = e1;
-                bnew = M.finite;
                 if (it < 0) atomicAdd(&c->j2cap, 1);
             }
         }
     }
-    if (bnew) {
-        // ε = tensor(Del \ τ, ½); b = exp(2ε) (DP.jl:63-66)
-        T y[NS];
-        lu_solve<NS, T>(M.lu, s, y);
-        T a[D][D], V[D][D];
-        if (D == 2) { a[0][0] = y[0]; a[1][1] = y[1]; a[0][1] = a[1][0] = T(0.5) * y[2]; }
-        else {
-            a[0][0] = y[0]; a[1][1] = y[1]; a[2][2] = y[2];
-            a[1][2] = a[2][1] = T(0.5) * y[3]; a[0][2] = a[2][0] = T(0.5) * y[4]; a[0][1] = a[1][0] = T(0.5) * y[5];
-        }
-        sym_eig<D, T>(a, V);
-        T fe[D];
-#pragma unroll
-        for (int k = 0; k < D; k++) fe[k] = exp(T(2) * a[k][k]);
-        T bn[D][D];
-#pragma unroll
-        for (int i = 0; i < D; i++)
-#pragma unroll
-            for (int j = i; j < D; j++) {
-                T acc = T(0);
-#pragma unroll
-                for (int k = 0; k < D; k++) acc += (V[i][k] * fe[k]) * V[j][k];
-                bn[i][j] = acc;
-            }
-        if (D == 2) { P.b[0][p] = bn[0][0]; P.b[1][p] = bn[1][1]; P.b[2][p] = bn[0][1]; }
-        else { P.b[0][p] = bn[0][0]; P.b[1][p] = bn[1][1]; P.b[2][p] = bn[2][2]; P.b[3][p] = bn[1][2]; P.b[4][p] = bn[0][2]; P.b[5][p] = bn[0][1]; }
-    }
-    bool bad = false;
 #pragma unroll
     for (int k = 0; k < NS; k++) {
         P.s[k][p] = s[k];
@@ -825,9 +948,56 @@ __device__ __forceinline__ void constitutive(Ctrl *c, const PState<T> &P, const 
     if (bad) atomicOr(&c->err, ERR_NONFINITE);
 }
 
-// finite/infinitesimal_deform (deform.jl:1-45) + (F-bar off) elast/plast,
-// or (F-bar on) ΔJn scatter (volumetric.jl:1-11)
-template <int D, int B, typename T>
+// ∇v gather (deform.jl:5-12) from a gather source, in the reference order
+template <int D, typename T, typename Src>
+__device__ __forceinline__ void grad_v(const Geom &g, const W<D, T> &w, const Src &src, int v0, T (&gv)[D][D]) {
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+        for (int j = 0; j < D; j++) gv[i][j] = T(0);
+    for_stencil<D, T>(g, w, [&](long no, T, const T *dN, const int *ix) {
+        const int li = src.local(ix);
+        T vn[D];
+#pragma unroll
+        for (int i = 0; i < D; i++) vn[i] = T(src(v0 + i, no, li));
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; j < D; j++) gv[i][j] += dN[j] * vn[i];
+    });
+}
+
+// ΔF = I + dt ∇v, F_new = ΔF F (deform.jl:14-15)
+template <int D, typename T>
+__device__ __forceinline__ void deform_f(const PState<T> &P, long q, T dt, const T (&gv)[D][D], T (&dF)[D][D],
+                                         T (&Fn)[D][D]) {
+    T F[D][D];
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+            // FP32 path: dF holds K = dt ∇v = ΔF - I
+            dF[i][j] = kDev<T> ? dt * gv[i][j] : (i == j ? T(1) : T(0)) + dt * gv[i][j];
+            F[i][j] = P.F[i + D * j][q];
+        }
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+            T a = dF[i][0] * F[0][j];
+#pragma unroll
+            for (int k = 1; k < D; k++) a += dF[i][k] * F[k][j];
+            Fn[i][j] = kDev<T> ? F[i][j] + a : a;
+        }
+}
+
+// finite/infinitesimal_deform (deform.jl:1-45).
+// LOCK = false: the whole particle update (deform + elast + plast) in one
+//   pass.
+// LOCK = true: only the ΔJn scatter of F-bar (volumetric.jl:1-11), which
+//   needs ΔJ = det ΔF and ρΩ = ρ₀/J·JΩ₀ of the updated J; nothing is
+//   written back, k_fbar recomputes ΔF from the same grid velocity.
+template <int D, int B, typename T, bool LOCK>
 __global__ void __launch_bounds__(256) k_deform(Ctrl *c, Geom g, PState<T> P, GState G, Mat M) {
     __shared__ double smem[ACC_LDS];
     __shared__ int sint[8];
@@ -836,7 +1006,8 @@ __global__ void __launch_bounds__(256) k_deform(Ctrl *c, Geom g, PState<T> P, GS
     const bool live = p < P.n;
     const long q = live ? p : 0;
     const int par = c->parity;
-    T xp[D], lp[D];
+    double xp[D];
+    T lp[D];
 #pragma unroll
     for (int i = 0; i < D; i++) { xp[i] = P.x[par][i][q]; lp[i] = B == 1 ? P.l0[i][q] : T(0); }
     W<D, T> w;
@@ -847,67 +1018,17 @@ __global__ void __launch_bounds__(256) k_deform(Ctrl *c, Geom g, PState<T> P, GS
     const bool staged = box.fits(D);
     if (staged) box.template stage<D>(g, G.v);
     T gv[D][D];
-#pragma unroll
-    for (int i = 0; i < D; i++)
-#pragma unroll
-        for (int j = 0; j < D; j++) gv[i][j] = T(0);
-    auto gather = [&](const auto &src) {
-        for_stencil<D, T>(g, w, [&](long no, T, const T *dN, const int *ix) {
-            const int li = src.local(ix);
-            T vn[D];
-#pragma unroll
-            for (int i = 0; i < D; i++) vn[i] = T(src(i, no, li));
-#pragma unroll
-            for (int i = 0; i < D; i++)
-#pragma unroll
-                for (int j = 0; j < D; j++) gv[i][j] += dN[j] * vn[i];
-        });
-    };
-    if (staged) gather(LdsSrc<D>{&box});
-    else gather(GlobSrc{G.v});
-    const T dt = T(c->dt);
-    T dF[D][D], F[D][D], Fn[D][D];
-#pragma unroll
-    for (int i = 0; i < D; i++)
-#pragma unroll
-        for (int j = 0; j < D; j++) {
-            dF[i][j] = (i == j ? T(1) : T(0)) + dt * gv[i][j];
-            F[i][j] = P.F[i + D * j][q];
-        }
-#pragma unroll
-    for (int i = 0; i < D; i++)
-#pragma unroll
-        for (int j = 0; j < D; j++) {
-            T a = dF[i][0] * F[0][j];
-#pragma unroll
-            for (int k = 1; k < D; k++) a += dF[i][k] * F[k][j];
-            Fn[i][j] = a;
-        }
-    const T dJ = det<D, T>(dF);
+    if (staged) grad_v<D, T>(g, w, LdsSrc<D>{&box}, 0, gv);
+    else grad_v<D, T>(g, w, GlobSrc{G.v}, 0, gv);
+    T dF[D][D], Fn[D][D];
+    deform_f<D, T>(P, q, T(c->dt), gv, dF, Fn);
     const T J = det<D, T>(Fn);
-    if (live) {
-#pragma unroll
-        for (int i = 0; i < D; i++)
-#pragma unroll
-            for (int j = 0; j < D; j++) {
-                P.F[i + D * j][p] = Fn[i][j];
-                if (M.tpic) P.gv[i + D * j][p] = gv[i][j];
-            }
-        P.J[p] = J;
-    }
-    if (M.locking) {
-        if (live) {
-#pragma unroll
-            for (int i = 0; i < D; i++)
-#pragma unroll
-                for (int j = 0; j < D; j++) P.dF[i + D * j][p] = dF[i][j];
-            P.dJ[p] = dJ;
-        }
+    if (LOCK) {
+        const T dJ = kDev<T> ? det1m<D, T>(dF) : det<D, T>(dF);   // FP32 path: δJ = ΔJ - 1
         const bool lds = box.fits(1);
         if (lds) box.zero(1);
         double *dst[1] = {G.dJ};
-        const T ms = mass_of(P, M, q, J);
-        const T qv = ms * dJ;
+        const T qv = mass_of(P, M, q, J) * dJ;
         auto scatter = [&](const auto &sink) {
             for_stencil_rot<D, T>(g, w, [&](long no, T N, const T *, const int *ix) {
                 sink(0, no, sink.local(ix), double(N * qv));
@@ -917,12 +1038,23 @@ __global__ void __launch_bounds__(256) k_deform(Ctrl *c, Geom g, PState<T> P, GS
         else scatter(GlobSink{dst});
         if (lds) box.template flush<1>(g, dst);
     } else if (live) {
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; j < D; j++) {
+                P.F[i + D * j][p] = Fn[i][j];
+                if (M.tpic) P.gv[i + D * j][p] = gv[i][j];
+            }
+        P.J[p] = J;
         constitutive<D, T>(c, P, M, p, dF);
     }
 }
 
-// ΔJs + ΔJp (volumetric.jl:12-35) fused, then elast/plast.  The staged
-// nodal value is ΔJ_n/m_n exactly as ΔJs writes it (0 where m_n = 0).
+// F-bar (volumetric.jl:12-35) then elast/plast.  The box stages the grid
+// velocity and ΔJ_n/m_n exactly as ΔJs writes it (0 where m_n = 0); ΔF,
+// ΔJ and F_new are recomputed from the staged velocity with the same
+// arithmetic as k_deform, F and J are written here, and ΔF is scaled by
+// (Σ N ΔJ_n/ΔJ_p)^(1/d) before the constitutive update.
 template <int D, int B, typename T>
 __global__ void __launch_bounds__(256) k_fbar(Ctrl *c, Geom g, PState<T> P, GState G, Mat M) {
     __shared__ double smem[ACC_LDS];
@@ -932,7 +1064,8 @@ __global__ void __launch_bounds__(256) k_fbar(Ctrl *c, Geom g, PState<T> P, GSta
     const bool live = p < P.n;
     const long q = live ? p : 0;
     const int par = c->parity;
-    T xp[D], lp[D];
+    double xp[D];
+    T lp[D];
 #pragma unroll
     for (int i = 0; i < D; i++) { xp[i] = P.x[par][i][q]; lp[i] = B == 1 ? P.l0[i][q] : T(0); }
     W<D, T> w;
@@ -940,38 +1073,67 @@ __global__ void __launch_bounds__(256) k_fbar(Ctrl *c, Geom g, PState<T> P, GSta
     if (!live) w.ok = 0u;
     BlockBox<D> box;
     box.begin((EP_LDS double *)smem, sint, live, w.base);
-    const bool staged = box.fits(1);
+    const bool staged = box.fits(D + 1);
     if (staged) {
         for (int k = threadIdx.x; k < box.nloc; k += blockDim.x) {
             const long no = box.node(g, k);
-            double val = 0.0;
+            double qv = 0.0;
             if (no >= 0) {
                 const double m = G.m[no];
-                val = m == 0.0 ? 0.0 : G.dJ[no] / m;
+                qv = m == 0.0 ? 0.0 : G.dJ[no] / m;
             }
-            box.buf[k] = val;
+#pragma unroll
+            for (int d = 0; d < D; d++) box.buf[d * box.nloc + k] = no >= 0 ? G.v[d][no] : 0.0;
+            box.buf[D * box.nloc + k] = qv;
         }
         __syncthreads();
     }
     if (!live) return;
-    const T dJp = P.dJ[p];
+    T gv[D][D];
+    if (staged) grad_v<D, T>(g, w, LdsSrc<D>{&box}, 0, gv);
+    else grad_v<D, T>(g, w, GlobSrc{G.v}, 0, gv);
+    T dF[D][D], Fn[D][D];
+    deform_f<D, T>(P, p, T(c->dt), gv, dF, Fn);
+    const T dJp = kDev<T> ? det1m<D, T>(dF) : det<D, T>(dF);
+    const T J = det<D, T>(Fn);
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+            P.F[i + D * j][p] = Fn[i][j];
+            if (M.tpic) P.gv[i + D * j][p] = gv[i][j];
+        }
+    P.J[p] = J;
     T acc = T(0);
+    // FP64: Σ N ΔJ_n/ΔJ_p as volumetric.jl:27-32.  FP32: the nodal values are
+    // δJ_n = ΔJ_n - 1 and Σ N δJ_n is accumulated, so that with Σ N = 1
+    // ΔJ̄/ΔJ_p - 1 = (Σ N δJ_n - δJ_p)/(1 + δJ_p) keeps its relative precision.
+    const T rdiv = kDev<T> ? T(1) : dJp;
     if (staged) {
         for_stencil<D, T>(g, w, [&](long, T N, const T *, const int *ix) {
-            acc += N * T(box.buf[box.local(ix)]) / dJp;
+            acc += N * T(box.buf[D * box.nloc + box.local(ix)]) / rdiv;
         });
     } else {
         for_stencil<D, T>(g, w, [&](long no, T N, const T *, const int *) {
             const double m = G.m[no];
-            acc += N * (m == 0.0 ? T(0) : T(G.dJ[no] / m)) / dJp;
+            acc += N * (m == 0.0 ? T(0) : T(G.dJ[no] / m)) / rdiv;
         });
     }
-    const T f = pow(acc, T(1.0 / D));
-    T dF[D][D];
+    if constexpr (kDev<T>) {
+        const T rm1 = (acc - dJp) / (T(1) + dJp);
+        const T fm1 = expm1(log1p(rm1) / T(D));   // (ΔJ̄/ΔJ_p)^(1/d) - 1
+        // K̄ = f (I + K) - I = K + (f - 1)(I + K)
 #pragma unroll
-    for (int i = 0; i < D; i++)
+        for (int i = 0; i < D; i++)
 #pragma unroll
-        for (int j = 0; j < D; j++) dF[i][j] = P.dF[i + D * j][p] * f;
+            for (int j = 0; j < D; j++) dF[i][j] = dF[i][j] + fm1 * ((i == j ? T(1) : T(0)) + dF[i][j]);
+    } else {
+        const T f = pow(acc, T(1.0 / D));
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; j < D; j++) dF[i][j] *= f;
+    }
     constitutive<D, T>(c, P, M, p, dF);
 }
 

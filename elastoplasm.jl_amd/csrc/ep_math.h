@@ -12,8 +12,8 @@ namespace ep {
 
 // Floor of the exact quotient a/h, which is what Julia's fld returns for
 // floats (tplgy.jl:47-48, 79-81); the fma residuals are exact in sign.
-__device__ __forceinline__ double exact_fld(double a, double h) {
-    double q = floor(a / h);
+__device__ __forceinline__ double exact_fld(double a, double h, double ih) {
+    double q = floor(a * ih);
     if (fma(-q, h, a) < 0.0) q -= 1.0;
     else if (fma(-(q + 1.0), h, a) >= 0.0) q += 1.0;
     return q;
