@@ -43,14 +43,24 @@ WORKLOADS = {
     "slump2d_40x10": ([64.1584, 16.0396], [40, 10], "bsmpm", None),
 }
 
-# Algorithmic bytes per particle for each kernel of the 3D finite / mUSL /
-# F-bar / DP FP64 step (8-byte values read + written that the kernel must
-# touch; grid traffic excluded).  Whole step: 496 B (B_alg).
+# Algorithmic bytes per particle for each particle kernel of the 3D finite /
+# mUSL / F-bar / Drucker-Prager step: the particle values the kernel must read
+# and write, 8 B each in FP64; in FP32 4 B each except the FP64 positions.
+# Grid traffic is excluded.  The whole step's minimum (each persistent value
+# read and written once per step) is B_ALG.
 KERNEL_BYTES_3D = {
-    "p2g": 8 * (3 + 3 + 6 + 1 + 1 + 1),          # x v tau J rho0 Om0
-    "g2p": 8 * ((3 + 3 + 1 + 1 + 1) + (3 + 3)),  # reads x v J rho0 Om0, writes x' v
-    "deform": 8 * ((3 + 9 + 1 + 1) + (9 + 1)),   # reads x F rho0 Om0, writes F J
-    "fbar": 8 * ((3 + 6 + 1 + 3) + (6 + 6 + 1 + 1)),  # reads x b epII c0 cr phi, writes b tau epII dlam
+    "fp64": {
+        "p2g": 8 * (3 + 3 + 6 + 1 + 1 + 1),                  # x v tau J rho0 Om0
+        "g2p": 8 * ((3 + 3 + 1 + 1 + 1) + (3 + 3)),          # reads x v J rho0 Om0, writes x' v
+        "deform": 8 * (3 + 9 + 1 + 1),                       # reads x F rho0 Om0 (scatters m*dJ to the grid)
+        "fbar": 8 * ((3 + 9 + 6 + 1 + 3) + (9 + 1 + 6 + 6 + 1 + 1)),  # reads x F b epII c0 cr phi,
+    },                                                        # writes F J b tau epII dlam
+    "fp32": {
+        "p2g": 8 * 3 + 4 * (3 + 6 + 1 + 1 + 1),
+        "g2p": 8 * (3 + 3) + 4 * ((3 + 1 + 1 + 1) + 3),
+        "deform": 8 * 3 + 4 * (9 + 1 + 1),
+        "fbar": 8 * 3 + 4 * ((9 + 6 + 1 + 3) + (9 + 1 + 6 + 6 + 1 + 1)),
+    },
 }
 B_ALG = {3: 496, 2: 288}
 
@@ -98,6 +108,23 @@ def cpu_baseline(name: str, budget_s: float, threads: int):
                       f"same column, {nmp} particles, {steps} steps in {el:.1f} s, OpenMP {threads} threads"}
 
 
+def measured_traffic(workload: str, precision: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes summarised in profiles/hbm_traffic_<workload>.md
+    (counters already scaled by the copy-kernel calibration); None when
+    that workload has not been profiled."""
+    f = ROOT / "profiles" / f"hbm_traffic_{workload}.md"
+    if not f.exists():
+        return None
+    txt = f.read_text()
+    try:
+        blob = txt[txt.index("```json") + 7:]
+        blob = blob[:blob.index("```")]
+        return json.loads(blob)[precision][kernel]["traffic_bytes"]
+    except (ValueError, KeyError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +133,8 @@ def main():
     ap.add_argument("--workload", default="slump3d_256x64x128", choices=list(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precision", default="both", choices=["both", "fp64", "fp32"],
+                    help="fp64 alone gives the headline line without the fp32 comparison (used under profilers)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the step would shard by spatial "
@@ -145,24 +174,38 @@ def main():
           flush=True)
     for k, v in sorted(per_kernel.items(), key=lambda kv: -kv[1]):
         print(f"[bench]   {k:10s} {v*1000:9.1f} us/launch", file=sys.stderr)
-    dom = max((k for k in per_kernel if k in KERNEL_BYTES_3D), key=lambda k: per_kernel[k])
-    kb = KERNEL_BYTES_3D[dom] * (1 if dim == 3 else 0.5)
+    kbytes = KERNEL_BYTES_3D["fp64"]
+    dom = max((k for k in per_kernel if k in kbytes), key=lambda k: per_kernel[k])
+    kb = kbytes[dom] * (1 if dim == 3 else 0.5)
     achieved = kb * nmp / (per_kernel[dom] * 1e-3) / 1e9
     step_ms = sum(per_kernel.values())
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
-            "kernel_ms": per_kernel[dom], "bytes_per_particle": kb}
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload, "fp64", dom),
+            "kernel": dom, "kernel_ms": per_kernel[dom], "bytes_per_particle": kb,
+            "algorithmic_bytes": kb * nmp}
     step_roof = {"bytes_per_particle": B_ALG[dim], "achieved": B_ALG[dim] * value / 1e9,
                  "frac": B_ALG[dim] * value / 1e9 / HBM_PEAK_GBS, "kernel_ms_per_step": step_ms}
     finite = bool(np.isfinite(st64.mpts.s.tau).all())
-    v32, el32, pk32, st32 = timed_run("fp32")
-    err32 = float(np.linalg.norm(st32.mpts.s.tau - st64.mpts.s.tau) / np.linalg.norm(st64.mpts.s.tau))
-    x0 = ic.mpts.x
-    errx32 = float(np.linalg.norm(st32.mpts.x - st64.mpts.x) / np.linalg.norm(st64.mpts.x - x0))
-    print(f"[bench] fp32: {1000*el32/args.steps:.3f} ms/step, {v32:.4g} pu/s, stress err vs fp64 {err32:.2e}",
-          file=sys.stderr, flush=True)
-    for k, v in sorted(pk32.items(), key=lambda kv: -kv[1]):
-        print(f"[bench]   fp32 {k:10s} {v*1000:9.1f} us/launch", file=sys.stderr)
+    fp32 = None
+    if args.precision in ("both", "fp32"):
+        v32, el32, pk32, st32 = timed_run("fp32")
+        err32 = float(np.linalg.norm(st32.mpts.s.tau - st64.mpts.s.tau) / np.linalg.norm(st64.mpts.s.tau))
+        x0 = ic.mpts.x
+        errx32 = float(np.linalg.norm(st32.mpts.x - st64.mpts.x) / np.linalg.norm(st64.mpts.x - x0))
+        print(f"[bench] fp32: {1000*el32/args.steps:.3f} ms/step, {v32:.4g} pu/s, stress err vs fp64 {err32:.2e}",
+              file=sys.stderr, flush=True)
+        for k, v in sorted(pk32.items(), key=lambda kv: -kv[1]):
+            print(f"[bench]   fp32 {k:10s} {v*1000:9.1f} us/launch", file=sys.stderr)
+        d32 = max((k for k in pk32 if k in KERNEL_BYTES_3D["fp32"]), key=lambda k: pk32[k])
+        kb32 = KERNEL_BYTES_3D["fp32"][d32] * (1 if dim == 3 else 0.5)
+        a32 = kb32 * nmp / (pk32[d32] * 1e-3) / 1e9
+        fp32 = {"value": v32, "ms_per_step": 1000.0 * el32 / args.steps, "speedup_vs_fp64": v32 / value,
+                "roofline": {"bound": "hbm", "achieved": a32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": a32 / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload, "fp32", d32),
+                             "kernel": d32, "kernel_ms": pk32[d32], "bytes_per_particle": kb32},
+                "rel_stress_err_vs_fp64": err32, "rel_displacement_err_vs_fp64": errx32,
+                "kernels_us": {k: v * 1000 for k, v in pk32.items()},
+                "dtype": "f32 particles, f64 grid and near-degenerate eigen-decompositions"}
     cpu = None
     if not args.no_cpu_baseline:
         thr = min(16, os.cpu_count() or 1)
@@ -186,10 +229,7 @@ def main():
         "roofline": roof,
         "step_roofline": step_roof,
         "kernels_us": {k: v * 1000 for k, v in per_kernel.items()},
-        "fp32": {"value": v32, "ms_per_step": 1000.0 * el32 / args.steps, "speedup_vs_fp64": v32 / value,
-                 "rel_stress_err_vs_fp64": err32, "rel_displacement_err_vs_fp64": errx32,
-                 "kernels_us": {k: v * 1000 for k, v in pk32.items()},
-                 "dtype": "f32 particles, f64 grid and near-degenerate eigen-decompositions"},
+        "fp32": fp32,
         "state_finite": finite,
         "cpu_baseline": cpu,
     }
