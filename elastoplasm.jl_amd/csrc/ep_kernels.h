@@ -41,8 +41,6 @@ struct PState {
     T *Om0, *c0, *cr, *phi;
     T *l0[3];                  // GIMP half lengths
     T *gv[9];                  // ∇v (TPIC state)
-    T *dF[9];                  // ΔF scratch between deform and F-bar gather
-    T *dJ;                     // ΔJ scratch
     long n;
 };
 

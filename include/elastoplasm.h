@@ -95,7 +95,7 @@ typedef struct ep_particles_view {
     double *rho0, *rho;   /* [nmp] mpts.s.ρ₀, mpts.s.ρ */
     double *Om0, *Om;     /* [nmp] mpts.Ω₀, mpts.Ω */
     double *n0, *n;       /* [nmp] mpts.n₀, mpts.n */
-    double *J, *dJ;       /* [nmp] mpts.J, mpts.ΔJ */
+    double *J, *dJ;       /* [nmp] mpts.J; mpts.ΔJ is ignored: ΔJ_p = det ΔF lives only inside a step */
     double *c0, *cr;      /* [nmp] mpts.s.c₀, mpts.s.cᵣ */
     double *phi;          /* [nmp] mpts.s.ϕ */
     double *dlam;         /* [nmp] mpts.s.Δλ */
