@@ -33,6 +33,7 @@ import elastoplasm as ep  # noqa: E402
 from elastoplasm import hip  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+SURR_TOL = 1e-4      # scaled residual a learned-return prediction must meet
 MAT = dict(rho0=2650.0, E=1.0e6, nu=0.3, c0=20.0e3, cr=20.0e3, phi0=math.radians(30.0), phir=math.radians(30.0),
            Hp=0.0)
 
@@ -133,8 +134,10 @@ def main():
     ap.add_argument("--workload", default="slump3d_256x64x128", choices=list(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--precision", default="both", choices=["both", "fp64", "fp32"],
-                    help="fp64 alone gives the headline line without the fp32 comparison (used under profilers)")
+    ap.add_argument("--precision", default="all", choices=["all", "fp64", "fp32", "fp32_surrogate"],
+                    help="fp64 alone gives the headline line without the comparisons (used under profilers)")
+    ap.add_argument("--surr-samples", type=int, default=10_000_000, help="training samples of the learned return")
+    ap.add_argument("--surr-steps", type=int, default=6000, help="Adam steps (batch 65536)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the step would shard by spatial "
@@ -148,13 +151,33 @@ def main():
     print(f"[bench] {args.workload}: nmp={nmp} nno={int(ic.mesh.nno[-1])} setup {time.perf_counter()-t_setup:.1f}s",
           file=sys.stderr, flush=True)
 
+    net = None
+    surr = None
+    if args.precision in ("all", "fp32_surrogate") and dim == 3:
+        # learned return: training samples and Adam training on the device
+        net = hip.Surrogate(seed=45)
+        t0 = time.perf_counter()
+        net.sample(args.surr_samples, 44)
+        t_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        loss = net.train(args.surr_steps, 65536, 2e-3, 2e-5)
+        t_t = time.perf_counter() - t0
+        print(f"[bench] surrogate: {args.surr_samples} samples in {t_s:.2f}s, {args.surr_steps} Adam steps "
+              f"(batch 65536) in {t_t:.2f}s, loss {loss[:10].mean():.3e} -> {loss[-50:].mean():.3e}",
+              file=sys.stderr, flush=True)
+        surr = {"train_samples": args.surr_samples, "train_steps": args.surr_steps, "batch": 65536,
+                "sample_s": t_s, "train_s": t_t, "train_samples_per_s": args.surr_steps * 65536 / t_t,
+                "loss_first": float(loss[:10].mean()), "loss_last": float(loss[-50:].mean())}
+
     def timed_run(precision):
         """warmup + timed steps from the initial state; returns (pu/s, s,
-        per-kernel {name: ms per launch}, final state)"""
+        per-kernel {name: ms per launch}, final state, stats)"""
         st = copy.deepcopy(ic)
         instr = copy.deepcopy(cfg.instr)
-        instr["ops"]["precision"] = {"fp64": 0, "fp32": 1}[precision]
+        instr["ops"]["precision"] = {"fp64": 0, "fp32": 1, "fp32_surrogate": 2}[precision]
         dev = hip.Device(st.mesh, st.mpts, st.cmpr, instr)
+        if precision == "fp32_surrogate":
+            dev.attach_surrogate(net, SURR_TOL)
         if args.warmup:
             dev.step(args.warmup, 1e9)
         dev.profile(True)
@@ -165,10 +188,11 @@ def main():
         dev.profile(False)
         assert r.steps == args.steps, "steps were skipped"
         dev.download()
+        stats = dev.stats()
         dev.close()
-        return nmp * args.steps / el, el, {k: ms / n for k, (ms, n) in kt.items()}, st
+        return nmp * args.steps / el, el, {k: ms / n for k, (ms, n) in kt.items()}, st, stats
 
-    value, el, per_kernel, st64 = timed_run("fp64")
+    value, el, per_kernel, st64, _ = timed_run("fp64")
     ms_step = 1000.0 * el / args.steps
     print(f"[bench] {args.steps} steps in {el:.3f}s: {ms_step:.3f} ms/step, {value:.4g} pu/s", file=sys.stderr,
           flush=True)
@@ -187,8 +211,8 @@ def main():
                  "frac": B_ALG[dim] * value / 1e9 / HBM_PEAK_GBS, "kernel_ms_per_step": step_ms}
     finite = bool(np.isfinite(st64.mpts.s.tau).all())
     fp32 = None
-    if args.precision in ("both", "fp32"):
-        v32, el32, pk32, st32 = timed_run("fp32")
+    if args.precision in ("all", "fp32"):
+        v32, el32, pk32, st32, _ = timed_run("fp32")
         err32 = float(np.linalg.norm(st32.mpts.s.tau - st64.mpts.s.tau) / np.linalg.norm(st64.mpts.s.tau))
         x0 = ic.mpts.x
         errx32 = float(np.linalg.norm(st32.mpts.x - st64.mpts.x) / np.linalg.norm(st64.mpts.x - x0))
@@ -206,6 +230,23 @@ def main():
                 "rel_stress_err_vs_fp64": err32, "rel_displacement_err_vs_fp64": errx32,
                 "kernels_us": {k: v * 1000 for k, v in pk32.items()},
                 "dtype": "f32 particles, f64 grid and near-degenerate eigen-decompositions"}
+    if net is not None:
+        vs, els, pks, sts, stt = timed_run("fp32_surrogate")
+        errs = float(np.linalg.norm(sts.mpts.s.tau - st64.mpts.s.tau) / np.linalg.norm(st64.mpts.s.tau))
+        x0 = ic.mpts.x
+        errxs = float(np.linalg.norm(sts.mpts.x - st64.mpts.x) / np.linalg.norm(st64.mpts.x - x0))
+        pred = int(stt.predictions)
+        print(f"[bench] fp32+surrogate: {1000*els/args.steps:.3f} ms/step, {vs:.4g} pu/s, stress err vs fp64 "
+              f"{errs:.2e}, predictions {pred}, fallbacks {int(stt.fallbacks)}", file=sys.stderr, flush=True)
+        for k, v in sorted(pks.items(), key=lambda kv: -kv[1]):
+            print(f"[bench]   surr {k:10s} {v*1000:9.1f} us/launch", file=sys.stderr)
+        surr.update({"value": vs, "ms_per_step": 1000.0 * els / args.steps, "speedup_vs_fp64": vs / value,
+                     "rel_stress_err_vs_fp64": errs, "rel_displacement_err_vs_fp64": errxs,
+                     "predictions": pred, "accepted": pred - int(stt.fallbacks),
+                     "fallback_fraction": (int(stt.fallbacks) / pred) if pred else None,
+                     "max_scaled_residual": float(stt.max_residual), "tol": SURR_TOL,
+                     "kernels_us": {k: v * 1000 for k, v in pks.items()}})
+        net.close()
     cpu = None
     if not args.no_cpu_baseline:
         thr = min(16, os.cpu_count() or 1)
@@ -230,6 +271,7 @@ def main():
         "step_roofline": step_roof,
         "kernels_us": {k: v * 1000 for k, v in per_kernel.items()},
         "fp32": fp32,
+        "fp32_surrogate": surr,
         "state_finite": finite,
         "cpu_baseline": cpu,
     }

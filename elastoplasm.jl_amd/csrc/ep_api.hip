@@ -5,7 +5,7 @@
 // brick-major order (brick_key) so that a block of neighbouring threads
 // touches a compact box of nodes; download undoes the permutation.
 #include "../../include/elastoplasm.h"
-#include "ep_kernels.h"
+#include "ep_mlp.h"
 
 #include <algorithm>
 #include <cmath>
@@ -34,8 +34,9 @@ int fail(int code, const std::string &msg) {
             return fail(EP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));        \
     } while (0)
 
-enum KernelId { K_BEGIN, K_ZERO, K_P2G, K_EULER, K_G2P, K_AUGV, K_DEFORM, K_FBAR, K_END, K_COUNT };
-const char *kKernelNames[K_COUNT] = {"begin", "zero_grid", "p2g", "euler", "g2p", "augv", "deform", "fbar", "end"};
+enum KernelId { K_BEGIN, K_ZERO, K_P2G, K_EULER, K_G2P, K_AUGV, K_DEFORM, K_FBAR, K_SURR, K_END, K_COUNT };
+const char *kKernelNames[K_COUNT] = {"begin", "zero_grid", "p2g", "euler", "g2p", "augv", "deform", "fbar",
+                                     "surrogate", "end"};
 
 // LU with partial pivoting of Del (same algorithm the device solve expects).
 void factor_del(int n, const double *A, DelLU &f) {
@@ -87,6 +88,10 @@ struct ep_ctx {
     std::vector<long> perm;          // device slot -> host particle index
     std::vector<double> n0_h, rho0_h, Om0_h;
     bool part_ready = false;
+    // learned return (EP_FP32_SURROGATE): queue of plastic trial states
+    std::vector<void *> q_allocs;
+    ep_surrogate *surr = nullptr;
+    float surr_tol = 0.f;
     // control
     Ctrl *ctrl = nullptr;
     double *dt_log = nullptr;
@@ -120,7 +125,11 @@ extern "C" int ep_create(const ep_config *cfg, ep_ctx **out) {
     if (cfg->plast < EP_PLAST_NONE || cfg->plast > EP_J2) return fail(EP_E_CONFIG, "InvalidReturnMapping");
     if (cfg->precision != EP_FP64 && cfg->precision != EP_FP32 && cfg->precision != EP_FP32_SURROGATE)
         return fail(EP_E_CONFIG, "unknown precision mode");
+    if (cfg->precision == EP_FP32_SURROGATE &&
+        (cfg->dim != 3 || cfg->deform != EP_FINITE || cfg->plast != EP_DP))
+        return fail(EP_E_CONFIG, "the learned return needs 3D finite deformation with Drucker-Prager");
     if (cfg->nmp <= 0) return fail(EP_E_ARG, "nmp must be positive");
+    if (cfg->nmp > 0x7fffffffL) return fail(EP_E_ARG, "nmp exceeds 2^31 - 1");
     for (int i = 0; i < cfg->dim; i++) {
         if (cfg->nel[i] < 1 || cfg->nno[i] < 2 || cfg->nno[i] > (1 << 24) || !(cfg->h[i] > 0.0))
             return fail(EP_E_ARG, "bad mesh extent");
@@ -136,6 +145,8 @@ extern "C" int ep_create(const ep_config *cfg, ep_ctx **out) {
     }
     Mat &M = c->mat;
     M.Kc = cfg->Kc; M.Gc = cfg->Gc; M.Hp = cfg->Hp; M.c = cfg->c;
+    M.E = 9.0 * cfg->Kc * cfg->Gc / (3.0 * cfg->Kc + cfg->Gc);
+    M.nu = (3.0 * cfg->Kc - 2.0 * cfg->Gc) / (2.0 * (3.0 * cfg->Kc + cfg->Gc));
     M.C_pf = cfg->C_pf; M.damping = cfg->damping;
     std::memcpy(M.Del, cfg->Del, sizeof(double) * c->nstr * c->nstr);
     factor_del(c->nstr, cfg->Del, M.lu);
@@ -176,6 +187,7 @@ extern "C" void ep_destroy(ep_ctx *c) {
     hipFree(c->type_block);
     hipFree(c->bc_dev);
     for (void *a : c->part_allocs) hipFree(a);
+    for (void *a : c->q_allocs) hipFree(a);
     hipFree(c->ctrl);
     hipFree(c->dt_log);
     hipFree(c->vmax_part);
@@ -397,6 +409,22 @@ extern "C" int ep_upload_particles(ep_ctx *c, const ep_particles_view *v) {
         const int r = c->fp32 ? upload_fields<float>(c, v, c->psf) : upload_fields<double>(c, v, c->ps);
         if (r != EP_OK) return r;
     }
+    if (c->cfg.precision == EP_FP32_SURROGATE) {
+        for (void *a : c->q_allocs) hipFree(a);
+        c->q_allocs.clear();
+        auto qa = [&](size_t bytes) -> void * {
+            void *a = nullptr;
+            if (hipMalloc(&a, bytes) != hipSuccess) return nullptr;
+            c->q_allocs.push_back(a);
+            return a;
+        };
+        SurrQ &q = c->mat.q;
+        q.idx = static_cast<int *>(qa(sizeof(int) * n));
+        bool ok = q.idx != nullptr;
+        for (int k = 0; k < 3; k++) ok = ok && (q.e[k] = static_cast<float *>(qa(sizeof(float) * n))) != nullptr;
+        for (int k = 0; k < 9; k++) ok = ok && (q.V[k] = static_cast<float *>(qa(sizeof(float) * n))) != nullptr;
+        if (!ok) return fail(EP_E_HIP, "hipMalloc of the learned-return queue failed");
+    }
     c->n0_h.assign(n, 0.1);
     if (v->n0) for (long p = 0; p < n; p++) c->n0_h[p] = v->n0[p];
     c->rho0_h.assign(v->rho0, v->rho0 + n);
@@ -569,7 +597,9 @@ static void timed(ep_ctx *c, int kid, L &&launch) {
     c->stats.launches++;
 }
 
-template <int D, int B, typename T>
+static mlp::SurrNet surr_net(ep_ctx *c);
+
+template <int D, int B, typename T, bool SURR>
 static void launch_step(ep_ctx *c, const PState<T> &P) {
     const int TB = EP_BLOCK;
     const long n = c->nmp, nno = c->gs.nno;
@@ -586,8 +616,14 @@ static void launch_step(ep_ctx *c, const PState<T> &P) {
     timed(c, K_G2P, [&] { EP_LAUNCH((k_g2p<D, B, T>), gp, TB, s, ct, g, P, G, M); });
     if (M.musl) timed(c, K_AUGV, [&] { EP_LAUNCH(k_augv<D>, gg, TB, s, ct, G); });
     if (M.locking) timed(c, K_DEFORM, [&] { EP_LAUNCH((k_deform<D, B, T, true>), gp, TB, s, ct, g, P, G, M); });
-    else timed(c, K_DEFORM, [&] { EP_LAUNCH((k_deform<D, B, T, false>), gp, TB, s, ct, g, P, G, M); });
-    if (M.locking) timed(c, K_FBAR, [&] { EP_LAUNCH((k_fbar<D, B, T>), gp, TB, s, ct, g, P, G, M); });
+    else timed(c, K_DEFORM, [&] { EP_LAUNCH((k_deform<D, B, T, false, SURR>), gp, TB, s, ct, g, P, G, M); });
+    if (M.locking) timed(c, K_FBAR, [&] { EP_LAUNCH((k_fbar<D, B, T, SURR>), gp, TB, s, ct, g, P, G, M); });
+    if constexpr (SURR) {
+        const mlp::SurrNet net = surr_net(c);
+        // a fixed grid that strides over the queue; its length is read on the device
+        const int nb = EP_BLOCK == 256 ? 512 : 4;
+        timed(c, K_SURR, [&] { EP_LAUNCH(mlp::k_surr, nb, EP_BLOCK, s, ct, P, M, net); });
+    }
     timed(c, K_END, [&] { EP_LAUNCH(k_end, 1, 4 * EP_BLOCK, s, ct); });
 }
 
@@ -595,25 +631,35 @@ template <typename T>
 static void dispatch_step_t(ep_ctx *c, const PState<T> &P) {
     const int D = c->dim, B = c->cfg.basis;
     if (D == 2) {
-        if (B == EP_BSMPM) launch_step<2, 0, T>(c, P);
-        else if (B == EP_GIMPM) launch_step<2, 1, T>(c, P);
-        else launch_step<2, 2, T>(c, P);
+        if (B == EP_BSMPM) launch_step<2, 0, T, false>(c, P);
+        else if (B == EP_GIMPM) launch_step<2, 1, T, false>(c, P);
+        else launch_step<2, 2, T, false>(c, P);
     } else {
-        if (B == EP_BSMPM) launch_step<3, 0, T>(c, P);
-        else if (B == EP_GIMPM) launch_step<3, 1, T>(c, P);
-        else launch_step<3, 2, T>(c, P);
+        if (B == EP_BSMPM) launch_step<3, 0, T, false>(c, P);
+        else if (B == EP_GIMPM) launch_step<3, 1, T, false>(c, P);
+        else launch_step<3, 2, T, false>(c, P);
     }
 }
 
 static void dispatch_step(ep_ctx *c) {
-    if (c->fp32) dispatch_step_t<float>(c, c->psf);
-    else dispatch_step_t<double>(c, c->ps);
+    if (c->cfg.precision == EP_FP32_SURROGATE) {
+        const int B = c->cfg.basis;
+        if (B == EP_BSMPM) launch_step<3, 0, float, true>(c, c->psf);
+        else if (B == EP_GIMPM) launch_step<3, 1, float, true>(c, c->psf);
+        else launch_step<3, 2, float, true>(c, c->psf);
+    } else if (c->fp32) {
+        dispatch_step_t<float>(c, c->psf);
+    } else {
+        dispatch_step_t<double>(c, c->ps);
+    }
 }
 
 extern "C" int ep_step(ep_ctx *c, int32_t nsteps, const ep_step_params *par, ep_step_result *res) {
     if (!c || !par) return fail(EP_E_ARG, "ep_step: null argument");
     if (!c->part_ready || !c->grid_ready) return fail(EP_E_STATE, "upload grid and particles first");
     if (nsteps < 0) return fail(EP_E_ARG, "nsteps must be >= 0");
+    if (c->cfg.precision == EP_FP32_SURROGATE && !c->surr)
+        return fail(EP_E_STATE, "precision EP_FP32_SURROGATE needs ep_attach_surrogate first");
     Ctrl h{};
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipMemcpy(&h, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
@@ -629,6 +675,9 @@ extern "C" int ep_step(ep_ctx *c, int32_t nsteps, const ep_step_params *par, ep_
     h.steps = 0;
     h.err = 0;
     h.j2cap = 0;
+    h.s_acc = 0;
+    h.s_fb = 0;
+    h.s_maxr = 0;
     if (par->record_dt) {
         if (c->dt_cap < nsteps) {
             hipFree(c->dt_log);
@@ -654,6 +703,13 @@ extern "C" int ep_step(ep_ctx *c, int32_t nsteps, const ep_step_params *par, ep_
     HIPCHK(hipMemcpy(&h, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
     c->stats.steps += h.steps;
     c->stats.j2_maxit += h.j2cap;
+    c->stats.predictions += int64_t(h.s_acc + h.s_fb);
+    c->stats.fallbacks += int64_t(h.s_fb);
+    {
+        float mr;
+        std::memcpy(&mr, &h.s_maxr, 4);
+        if (double(mr) > c->stats.max_residual) c->stats.max_residual = double(mr);
+    }
     if (res) {
         res->t = h.t;
         res->last_dt = h.dt;
@@ -704,5 +760,254 @@ extern "C" int ep_kernel_times(ep_ctx *c, int32_t max_kernels, char *names, int3
 extern "C" int ep_get_stats(ep_ctx *c, ep_stats *st) {
     if (!c || !st) return fail(EP_E_ARG, "null argument");
     *st = c->stats;
+    return EP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// learned return mapping: network, training samples, training
+// ---------------------------------------------------------------------------
+struct ep_surrogate {
+    hipStream_t stream = nullptr;
+    float *prm = nullptr, *am = nullptr, *av = nullptr, *grad = nullptr;
+    unsigned short *img_f = nullptr, *img_b = nullptr;
+    long adam_t = 0;
+    long ns = 0;                                   // device training set
+    float *X = nullptr, *Y = nullptr;
+    double *raw = nullptr;
+    long bcap = 0;                                 // training scratch for batches up to bcap
+    int nsplit = 0;
+    unsigned short *act = nullptr, *del = nullptr;
+    float *lossp = nullptr, *part = nullptr;
+    float *loss = nullptr;
+    long loss_cap = 0;
+};
+
+static mlp::SurrNet surr_net(ep_ctx *c) { return mlp::SurrNet{c->surr->img_f, c->surr->prm, c->surr_tol}; }
+
+namespace {
+constexpr int kMlpBlock = EP_BLOCK;   // kernels with wave intrinsics: one wave per block in the CPU check
+
+void surr_free_train(ep_surrogate *s) {
+    hipFree(s->act); hipFree(s->del); hipFree(s->lossp); hipFree(s->part);
+    s->act = s->del = nullptr;
+    s->lossp = s->part = nullptr;
+    s->bcap = 0;
+}
+
+// number of batch splits of the weight-gradient product: a power of two up to
+// 64 that leaves every split a multiple of 32 samples
+int surr_splits(long batch) {
+    int S = 64;
+    while (S > 1 && (batch % (32L * S) != 0 || batch / S < 256)) S >>= 1;
+    return S;
+}
+
+int surr_ensure(ep_surrogate *s, long batch) {
+    const int S = surr_splits(batch);
+    if (batch <= s->bcap && S <= s->nsplit) return EP_OK;
+    surr_free_train(s);
+    HIPCHK(hipMalloc(&s->act, sizeof(unsigned short) * mlp::ACT_ROWS * batch));
+    HIPCHK(hipMalloc(&s->del, sizeof(unsigned short) * mlp::DEL_ROWS * batch));
+    HIPCHK(hipMalloc(&s->lossp, sizeof(float) * (batch / 32)));
+    HIPCHK(hipMalloc(&s->part, sizeof(float) * size_t(S) * mlp::NPARAM));
+    s->bcap = batch;
+    s->nsplit = S;
+    return EP_OK;
+}
+
+// forward + backward + weight gradient of one batch (and an Adam step when
+// a.update); the batch loss goes to *loss_dev
+void surr_batch(ep_surrogate *s, const float *X, const float *Y, long first, long batch, mlp::AdamArgs a,
+                float *loss_dev) {
+    const int S = surr_splits(batch);
+    mlp::TrainBuf tb{X, Y, first, batch, s->act, s->del, s->lossp};
+    const long ntile = batch / 32;
+    const long per = kMlpBlock / 64;
+    const int gb = int(std::min<long>((ntile + per - 1) / per, 1024));
+    EP_LAUNCH(mlp::k_mlp_fb, gb, kMlpBlock, s->stream, s->img_f, s->img_b, s->prm, tb);
+    EP_LAUNCH(mlp::k_mlp_dw, mlp::DW_TILES * S, 64, s->stream, s->act, s->del, batch, S, s->part);
+    EP_LAUNCH(mlp::k_adam, (mlp::NPARAM + 255) / 256, 256, s->stream, s->part, S, s->grad, s->prm, s->am, s->av,
+              s->img_f, s->img_b, a);
+    EP_LAUNCH(mlp::k_loss_sum, 1, 256, s->stream, s->lossp, ntile, batch, loss_dev);
+}
+}  // namespace
+
+extern "C" int ep_surrogate_create(uint64_t seed, ep_surrogate **out) {
+    if (!out) return fail(EP_E_ARG, "ep_surrogate_create: null argument");
+    *out = nullptr;
+    ep_surrogate *s = new ep_surrogate();
+    auto bad = [&](const char *m) { ep_surrogate_destroy(s); return fail(EP_E_HIP, m); };
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return bad("hipStreamCreate failed");
+    const size_t pb = sizeof(float) * mlp::NPARAM;
+    if (hipMalloc(&s->prm, pb) != hipSuccess || hipMalloc(&s->am, pb) != hipSuccess ||
+        hipMalloc(&s->av, pb) != hipSuccess || hipMalloc(&s->grad, pb) != hipSuccess ||
+        hipMalloc(&s->img_f, sizeof(unsigned short) * mlp::IF_N) != hipSuccess ||
+        hipMalloc(&s->img_b, sizeof(unsigned short) * mlp::IB_N) != hipSuccess ||
+        hipMalloc(&s->loss, sizeof(float)) != hipSuccess)
+        return bad("hipMalloc of the network failed");
+    s->loss_cap = 1;
+    // the padding of the swizzled images stays zero
+    if (hipMemset(s->img_f, 0, sizeof(unsigned short) * mlp::IF_N) != hipSuccess ||
+        hipMemset(s->img_b, 0, sizeof(unsigned short) * mlp::IB_N) != hipSuccess)
+        return bad("hipMemset failed");
+    // U(-1/sqrt(fan_in), 1/sqrt(fan_in)) for weights and biases of each layer
+    std::vector<float> w(mlp::NPARAM);
+    for (int k = 0; k < mlp::NPARAM; k++) {
+        const int fan = k < mlp::OW2 ? mlp::NI : mlp::NH;
+        const double u = mlp::u01(seed, (unsigned long long)k, 0);
+        w[k] = float((2.0 * u - 1.0) / std::sqrt(double(fan)));
+    }
+    *out = s;
+    return ep_surrogate_set_params(s, w.data());
+}
+
+extern "C" void ep_surrogate_destroy(ep_surrogate *s) {
+    if (!s) return;
+    if (s->stream) hipStreamSynchronize(s->stream);
+    surr_free_train(s);
+    hipFree(s->prm); hipFree(s->am); hipFree(s->av); hipFree(s->grad);
+    hipFree(s->img_f); hipFree(s->img_b);
+    hipFree(s->X); hipFree(s->Y); hipFree(s->raw); hipFree(s->loss);
+    if (s->stream) hipStreamDestroy(s->stream);
+    delete s;
+}
+
+extern "C" int ep_surrogate_set_params(ep_surrogate *s, const float *params) {
+    if (!s || !params) return fail(EP_E_ARG, "ep_surrogate_set_params: null argument");
+    for (int k = 0; k < mlp::NPARAM; k++)
+        if (!std::isfinite(params[k])) return fail(EP_E_ARG, "non-finite network parameter");
+    const size_t pb = sizeof(float) * mlp::NPARAM;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipMemcpy(s->prm, params, pb, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(s->am, 0, pb));
+    HIPCHK(hipMemset(s->av, 0, pb));
+    s->adam_t = 0;
+    EP_LAUNCH(mlp::k_mlp_pack, (mlp::NPARAM + 255) / 256, 256, s->stream, s->prm, s->img_f, s->img_b);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return EP_OK;
+}
+
+extern "C" int ep_surrogate_get_params(ep_surrogate *s, float *params) {
+    if (!s || !params) return fail(EP_E_ARG, "ep_surrogate_get_params: null argument");
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipMemcpy(params, s->prm, sizeof(float) * mlp::NPARAM, hipMemcpyDeviceToHost));
+    return EP_OK;
+}
+
+extern "C" int ep_surrogate_sample(ep_surrogate *s, int64_t n, uint64_t seed) {
+    if (!s) return fail(EP_E_ARG, "null surrogate");
+    if (n <= 0) return fail(EP_E_ARG, "n must be positive");
+    HIPCHK(hipStreamSynchronize(s->stream));
+    hipFree(s->X); hipFree(s->Y); hipFree(s->raw);
+    s->X = s->Y = nullptr;
+    s->raw = nullptr;
+    s->ns = 0;
+    HIPCHK(hipMalloc(&s->X, sizeof(float) * 8 * n));
+    HIPCHK(hipMalloc(&s->Y, sizeof(float) * 4 * n));
+    HIPCHK(hipMalloc(&s->raw, sizeof(double) * 6 * n));
+    s->ns = n;
+    EP_LAUNCH(mlp::k_mlp_samples, (n + 255) / 256, 256, s->stream, long(n), (unsigned long long)seed, s->X, s->Y,
+              s->raw);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return EP_OK;
+}
+
+extern "C" int ep_surrogate_get_samples(ep_surrogate *s, int64_t first, int64_t n, float *x, float *y, double *raw) {
+    if (!s) return fail(EP_E_ARG, "null surrogate");
+    if (first < 0 || n < 0 || first + n > s->ns) return fail(EP_E_ARG, "sample range outside the training set");
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (x) HIPCHK(hipMemcpy(x, s->X + 8 * first, sizeof(float) * 8 * n, hipMemcpyDeviceToHost));
+    if (y) HIPCHK(hipMemcpy(y, s->Y + 4 * first, sizeof(float) * 4 * n, hipMemcpyDeviceToHost));
+    if (raw) HIPCHK(hipMemcpy(raw, s->raw + 6 * first, sizeof(double) * 6 * n, hipMemcpyDeviceToHost));
+    return EP_OK;
+}
+
+extern "C" int ep_surrogate_train(ep_surrogate *s, int64_t steps, int64_t batch, double lr0, double lr1,
+                                  float *loss) {
+    if (!s) return fail(EP_E_ARG, "null surrogate");
+    if (steps < 0 || batch <= 0 || batch % 32 != 0) return fail(EP_E_ARG, "batch must be a positive multiple of 32");
+    if (s->ns < batch) return fail(EP_E_STATE, "ep_surrogate_sample a training set of at least one batch first");
+    int r;
+    if ((r = surr_ensure(s, batch)) != EP_OK) return r;
+    if (s->loss_cap < steps) {
+        hipFree(s->loss);
+        s->loss = nullptr;
+        HIPCHK(hipMalloc(&s->loss, sizeof(float) * std::max<int64_t>(steps, 1)));
+        s->loss_cap = std::max<int64_t>(steps, 1);
+    }
+    const long nb = s->ns / batch;
+    const double b1 = 0.9, b2 = 0.999;
+    for (int64_t i = 0; i < steps; i++) {
+        s->adam_t++;
+        const double lr = lr1 + 0.5 * (lr0 - lr1) * (1.0 + std::cos(3.14159265358979323846 * double(i) / double(steps)));
+        mlp::AdamArgs a{float(lr), float(b1), float(b2), 1e-8f, float(1.0 / (1.0 - std::pow(b1, double(s->adam_t)))),
+                        float(1.0 / (1.0 - std::pow(b2, double(s->adam_t)))), 1};
+        surr_batch(s, s->X, s->Y, (i % nb) * batch, batch, a, s->loss + i);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (loss && steps > 0) HIPCHK(hipMemcpy(loss, s->loss, sizeof(float) * steps, hipMemcpyDeviceToHost));
+    return EP_OK;
+}
+
+extern "C" int ep_surrogate_grad(ep_surrogate *s, const float *x, const float *y, int64_t n, float *grad,
+                                 float *loss) {
+    if (!s || !x || !y || !grad) return fail(EP_E_ARG, "ep_surrogate_grad: null argument");
+    if (n <= 0 || n % 32 != 0) return fail(EP_E_ARG, "n must be a positive multiple of 32");
+    int r;
+    if ((r = surr_ensure(s, n)) != EP_OK) return r;
+    float *dx = nullptr, *dy = nullptr;
+    HIPCHK(hipMalloc(&dx, sizeof(float) * 8 * n));
+    if (hipMalloc(&dy, sizeof(float) * 4 * n) != hipSuccess) { hipFree(dx); return fail(EP_E_HIP, "hipMalloc failed"); }
+    int rc = EP_OK;
+    if (hipMemcpy(dx, x, sizeof(float) * 8 * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dy, y, sizeof(float) * 4 * n, hipMemcpyHostToDevice) != hipSuccess) {
+        rc = fail(EP_E_HIP, "hipMemcpy failed");
+    } else {
+        mlp::AdamArgs a{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0};
+        surr_batch(s, dx, dy, 0, n, a, s->loss);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s->stream) != hipSuccess ||
+            hipMemcpy(grad, s->grad, sizeof(float) * mlp::NPARAM, hipMemcpyDeviceToHost) != hipSuccess ||
+            (loss && hipMemcpy(loss, s->loss, sizeof(float), hipMemcpyDeviceToHost) != hipSuccess))
+            rc = fail(EP_E_HIP, "gradient evaluation failed");
+    }
+    hipFree(dx);
+    hipFree(dy);
+    return rc;
+}
+
+extern "C" int ep_surrogate_predict(ep_surrogate *s, const float *x, int64_t n, float *y) {
+    if (!s || !x || !y) return fail(EP_E_ARG, "ep_surrogate_predict: null argument");
+    if (n <= 0) return fail(EP_E_ARG, "n must be positive");
+    const long np = (n + 31) / 32 * 32;
+    float *dx = nullptr, *dy = nullptr;
+    HIPCHK(hipMalloc(&dx, sizeof(float) * 8 * np));
+    if (hipMalloc(&dy, sizeof(float) * 4 * np) != hipSuccess) { hipFree(dx); return fail(EP_E_HIP, "hipMalloc failed"); }
+    int rc = EP_OK;
+    if (hipMemset(dx, 0, sizeof(float) * 8 * np) != hipSuccess ||
+        hipMemcpy(dx, x, sizeof(float) * 8 * n, hipMemcpyHostToDevice) != hipSuccess) {
+        rc = fail(EP_E_HIP, "hipMemcpy failed");
+    } else {
+        const long per = kMlpBlock / 64;
+        const int gb = int(std::min<long>((np / 32 + per - 1) / per, 1024));
+        EP_LAUNCH(mlp::k_mlp_predict, gb, kMlpBlock, s->stream, s->img_f, s->prm, dx, np, dy);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s->stream) != hipSuccess ||
+            hipMemcpy(y, dy, sizeof(float) * 4 * n, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(EP_E_HIP, "prediction failed");
+    }
+    hipFree(dx);
+    hipFree(dy);
+    return rc;
+}
+
+extern "C" int ep_attach_surrogate(ep_ctx *c, ep_surrogate *s, double tol) {
+    if (!c) return fail(EP_E_ARG, "null context");
+    if (c->cfg.precision != EP_FP32_SURROGATE) return fail(EP_E_CONFIG, "the context's precision is not EP_FP32_SURROGATE");
+    if (s) HIPCHK(hipStreamSynchronize(s->stream));
+    c->surr = s;
+    const double t = tol > 0.0 ? tol : (c->cfg.surrogate_tol > 0.0 ? c->cfg.surrogate_tol : 1e-3);
+    c->surr_tol = float(t);
     return EP_OK;
 }

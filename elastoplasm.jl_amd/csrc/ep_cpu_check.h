@@ -10,6 +10,7 @@
 // part of libelastoplasm_hip.so.
 #pragma once
 #include <atomic>
+#include <algorithm>
 #include <barrier>
 #include <chrono>
 #include <cmath>
@@ -40,13 +41,17 @@ struct dim3 {
 struct EmuIdx {
     unsigned x = 0, y = 0, z = 0;
 };
-inline thread_local EmuIdx threadIdx, blockIdx, blockDim;
+inline thread_local EmuIdx threadIdx, blockIdx, blockDim, gridDim;
 struct EmuBlock {
     std::barrier<> *bar = nullptr;
     unsigned long long *shfl = nullptr;
+    unsigned char *alive = nullptr;
+    unsigned long long *mf_a = nullptr, *mf_b = nullptr;
 };
 inline thread_local EmuBlock emu_block;
 
+// A thread that has returned from the kernel drops out of later barriers, as
+// an exited wave does on the GPU.
 inline void __syncthreads() { emu_block.bar->arrive_and_wait(); }
 
 inline void unsafeAtomicAdd(double *a, double v) { std::atomic_ref<double>(*a).fetch_add(v); }
@@ -78,6 +83,8 @@ inline int atomicMin(int *a, int v) { return emu_atomic_min(a, v); }
 inline int atomicMax(int *a, int v) { return emu_atomic_max(a, v); }
 inline unsigned long long atomicMax(unsigned long long *a, unsigned long long v) { return emu_atomic_max(a, v); }
 inline unsigned atomicOr(unsigned *a, unsigned v) { return std::atomic_ref<unsigned>(*a).fetch_or(v); }
+inline unsigned atomicMax(unsigned *a, unsigned v) { return emu_atomic_max(a, v); }
+typedef unsigned v4u __attribute__((vector_size(16)));
 inline long long __double_as_longlong(double d) {
     long long r;
     std::memcpy(&r, &d, 8);
@@ -88,7 +95,9 @@ inline double __longlong_as_double(long long l) {
     std::memcpy(&r, &l, 8);
     return r;
 }
-// Lane exchange inside a 64-wide wavefront; every thread of the block calls it.
+// Lane exchange inside a 64-wide wavefront; every running thread of the block
+// calls it (the kernels that use the wave intrinsics below run one wave per
+// block in this build).
 inline unsigned long long __shfl_xor(unsigned long long v, int mask) {
     const unsigned t = threadIdx.x;
     emu_block.shfl[t] = v;
@@ -97,6 +106,68 @@ inline unsigned long long __shfl_xor(unsigned long long v, int mask) {
     const unsigned long long r = src < blockDim.x ? emu_block.shfl[src] : v;
     __syncthreads();
     return r;
+}
+inline unsigned __lane_id() { return threadIdx.x & 63u; }
+inline int __popcll(unsigned long long v) { return __builtin_popcountll(v); }
+inline int __ffsll(unsigned long long v) { return __builtin_ffsll((long long)v); }
+// Lanes that have exited contribute 0, as on the GPU.
+inline unsigned long long __ballot(int pred) {
+    const unsigned t = threadIdx.x;
+    emu_block.shfl[t] = pred ? 1ull : 0ull;
+    __syncthreads();
+    const unsigned base = t & ~63u;
+    unsigned long long m = 0;
+    for (unsigned w = 0; w < 64 && base + w < blockDim.x; w++)
+        if (emu_block.alive[base + w] && emu_block.shfl[base + w]) m |= 1ull << w;
+    __syncthreads();
+    return m;
+}
+template <class T>
+inline T __shfl(T v, int src) {
+    static_assert(sizeof(T) <= 8, "shfl of more than 8 bytes");
+    const unsigned t = threadIdx.x;
+    unsigned long long u = 0;
+    std::memcpy(&u, &v, sizeof(T));
+    emu_block.shfl[t] = u;
+    __syncthreads();
+    const unsigned long long r = emu_block.shfl[(t & ~63u) + (unsigned(src) & 63u)];
+    __syncthreads();
+    T out;
+    std::memcpy(&out, &r, sizeof(T));
+    return out;
+}
+
+// v_mfma_f32_32x32x8_bf16 on one wave: lane l holds A[i = l%32][k = 4(l/32)+v]
+// and B[k = 4(l/32)+v][j = l%32] (v = 0..3); D[i][j] is returned in lane
+// j + 32*((i/4)%2), element 4*(i/8) + i%4.
+typedef short v4s __attribute__((vector_size(8)));
+typedef float v16f __attribute__((vector_size(64)));
+inline float emu_bf16(short h) {
+    const unsigned u = unsigned((unsigned short)h) << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+inline v16f emu_mfma_32x32x8bf16(v4s a, v4s b, v16f c) {
+    const unsigned t = threadIdx.x;
+    std::memcpy(&emu_block.mf_a[t], &a, 8);
+    std::memcpy(&emu_block.mf_b[t], &b, 8);
+    __syncthreads();
+    const unsigned base = t & ~63u, lane = t & 63u, j = lane & 31u, hh = lane >> 5;
+    v16f d;
+    for (int v = 0; v < 16; v++) {
+        const unsigned i = 8u * unsigned(v >> 2) + 4u * hh + unsigned(v & 3);
+        float acc = 0.f;
+        for (unsigned k = 0; k < 8; k++) {
+            short as[4], bs[4];
+            std::memcpy(as, &emu_block.mf_a[base + i + 32u * (k >> 2)], 8);
+            std::memcpy(bs, &emu_block.mf_b[base + j + 32u * (k >> 2)], 8);
+            acc += emu_bf16(as[k & 3]) * emu_bf16(bs[k & 3]);
+        }
+        d[v] = c[v] + acc;
+    }
+    __syncthreads();
+    return d;
 }
 
 // ---- runtime ----
@@ -165,19 +236,24 @@ inline const char *hipGetErrorString(hipError_t) { return "cpu check build"; }
 // __syncthreads.
 struct EmuTeam {
     unsigned n;
-    std::barrier<> start, done, bar;
-    std::vector<unsigned long long> shfl;
+    std::barrier<> start, done;
+    std::unique_ptr<std::barrier<>> bar;   // fresh per block: exited threads drop out of it
+    std::vector<unsigned long long> shfl, mf_a, mf_b;
+    std::vector<unsigned char> alive;
     std::vector<std::thread> th;
     std::function<void(unsigned)> job;
     bool quit = false;
-    explicit EmuTeam(unsigned n_) : n(n_), start(n_ + 1), done(n_ + 1), bar(n_), shfl(n_) {
+    explicit EmuTeam(unsigned n_)
+        : n(n_), start(n_ + 1), done(n_ + 1), shfl(n_), mf_a(n_), mf_b(n_), alive(n_) {
         for (unsigned t = 0; t < n; t++)
             th.emplace_back([this, t] {
                 for (;;) {
                     start.arrive_and_wait();
                     if (quit) return;
-                    emu_block = {&bar, shfl.data()};
+                    emu_block = {bar.get(), shfl.data(), alive.data(), mf_a.data(), mf_b.data()};
                     job(t);
+                    alive[t] = 0;
+                    bar->arrive_and_drop();
                     done.arrive_and_wait();
                 }
             });
@@ -189,6 +265,8 @@ struct EmuTeam {
     }
     void run(const std::function<void(unsigned)> &f) {
         job = f;
+        bar.reset(new std::barrier<>(n));
+        std::fill(alive.begin(), alive.end(), 1);
         start.arrive_and_wait();
         done.arrive_and_wait();
     }
@@ -211,6 +289,7 @@ inline void emu_launch(void (*k)(P...), dim3 g, dim3 b, A... args) {
             threadIdx = {t, 0, 0};
             blockIdx = {bx, 0, 0};
             blockDim = {b.x, 1, 1};
+            gridDim = {g.x, 1, 1};
             k(args...);
         });
     }

@@ -62,15 +62,30 @@ struct Ctrl {
     long dt_cap;
     unsigned long long *vmax_part;  // [3][nblk] per-block maxima of |v_p| written by G2P
     int nblk;
+    // learned return mapping (FP32 + surrogate path)
+    int qcount;                     // plastic trial states queued this step
+    unsigned long long s_acc, s_fb; // predictions accepted / replaced by the FP64 return
+    unsigned s_maxr;                // bit pattern of the largest scaled residual
 };
 
 enum : unsigned { ERR_DOMAIN = 1u, ERR_NONFINITE = 2u };
 
+// Queue of plastic trial states for the learned return (capacity nmp):
+// particle slot, principal log strains sorted descending and the matching
+// eigenvectors, V[i + 3k] = component i of eigenvector k.
+struct SurrQ {
+    int *idx;
+    float *e[3];
+    float *V[9];
+};
+
 struct Mat {
     double Kc, Gc, Hp, c, C_pf, damping;
+    double E, nu;
     double Del[36];
     DelLU lu;
     int finite, tpic, musl, locking, plast;
+    SurrQ q;
 };
 
 // -------------------------------------------------------------------------
@@ -414,6 +429,7 @@ template <int D>
 __global__ void k_begin(Ctrl *c, Geom g, Mat M) {
     if (threadIdx.x != 0) return;
     const double t = c->t;
+    c->qcount = 0;
     c->active = (c->T > t) ? 1 : 0;
     if (!c->active) return;
     double dt;
@@ -814,7 +830,7 @@ __device__ __forceinline__ void sym_to_voigt(const T (&m)[D][D], T *s) {
 // decomposition of ε_new yields the same matrix up to rounding.  The J2
 // cutting plane weights shear with Del's G against the 2·s² of J₂, which is
 // not coaxial, so J2 keeps the reference's Voigt-space iteration.
-template <int D, typename T>
+template <int D, typename T, bool SURR = false>
 __device__ __forceinline__ void constitutive(Ctrl *c, const PState<T> &P, const Mat &M, long p, const T (&dF)[D][D]) {
     constexpr int NS = D == 2 ? 3 : 6;
     T s[NS];
@@ -862,6 +878,41 @@ __device__ __forceinline__ void constitutive(Ctrl *c, const PState<T> &P, const 
             sp[i] = acc;
         }
         const bool plast = c->with_plast && M.plast != 0;
+        if constexpr (SURR) {
+            // Plastic trial states go to the learned return (k_surr), which
+            // writes τ, b, ϵpII and Δλ; the queue slots are reserved with one
+            // atomic per wave.
+            const bool pl = plast && M.plast == 1 &&
+                            dp_trial<NS, T>(sp, P.c0[p], P.cr[p], P.phi[p], T(M.Hp), P.epII[p]) != 0;
+            const unsigned long long mask = __ballot(pl);
+            if (mask) {
+                const int lane = int(__lane_id());
+                const int leader = __ffsll(mask) - 1;
+                int base = 0;
+                if (lane == leader) base = atomicAdd(&c->qcount, __popcll(mask));
+                base = __shfl(base, leader);
+                if (pl) {
+                    const int slot = base + __popcll(mask & ((1ull << lane) - 1ull));
+                    int o[D];
+#pragma unroll
+                    for (int k = 0; k < D; k++) o[k] = k;
+                    // principal values sorted descending, eigenvectors alike
+#pragma unroll
+                    for (int a0 = 0; a0 < D; a0++)
+#pragma unroll
+                        for (int b0 = 0; b0 + 1 < D - a0; b0++)
+                            if (ep[o[b0]] < ep[o[b0 + 1]]) { const int t = o[b0]; o[b0] = o[b0 + 1]; o[b0 + 1] = t; }
+                    M.q.idx[slot] = int(p);
+#pragma unroll
+                    for (int k = 0; k < D; k++) {
+                        M.q.e[k][slot] = float(ep[o[k]]);
+#pragma unroll
+                        for (int i = 0; i < D; i++) M.q.V[i + 3 * k][slot] = float(V[i][o[k]]);
+                    }
+                    return;
+                }
+            }
+        }
         if (plast && M.plast == 1) {
             T e2 = P.epII[p], e1 = e2, dl;
             dp_return<NS, T>(sp, P.c0[p], P.cr[p], P.phi[p], T(M.Hp), T(M.Kc), T(M.Gc), e2, e1, dl);
@@ -995,7 +1046,7 @@ __device__ __forceinline__ void deform_f(const PState<T> &P, long q, T dt, const
 // LOCK = true: only the ΔJn scatter of F-bar (volumetric.jl:1-11), which
 //   needs ΔJ = det ΔF and ρΩ = ρ₀/J·JΩ₀ of the updated J; nothing is
 //   written back, k_fbar recomputes ΔF from the same grid velocity.
-template <int D, int B, typename T, bool LOCK>
+template <int D, int B, typename T, bool LOCK, bool SURR = false>
 __global__ void __launch_bounds__(256) k_deform(Ctrl *c, Geom g, PState<T> P, GState G, Mat M) {
     __shared__ double smem[ACC_LDS];
     __shared__ int sint[8];
@@ -1044,7 +1095,7 @@ __global__ void __launch_bounds__(256) k_deform(Ctrl *c, Geom g, PState<T> P, GS
                 if (M.tpic) P.gv[i + D * j][p] = gv[i][j];
             }
         P.J[p] = J;
-        constitutive<D, T>(c, P, M, p, dF);
+        constitutive<D, T, SURR>(c, P, M, p, dF);
     }
 }
 
@@ -1053,7 +1104,7 @@ __global__ void __launch_bounds__(256) k_deform(Ctrl *c, Geom g, PState<T> P, GS
 // ΔJ and F_new are recomputed from the staged velocity with the same
 // arithmetic as k_deform, F and J are written here, and ΔF is scaled by
 // (Σ N ΔJ_n/ΔJ_p)^(1/d) before the constitutive update.
-template <int D, int B, typename T>
+template <int D, int B, typename T, bool SURR = false>
 __global__ void __launch_bounds__(256) k_fbar(Ctrl *c, Geom g, PState<T> P, GState G, Mat M) {
     __shared__ double smem[ACC_LDS];
     __shared__ int sint[8];
@@ -1132,7 +1183,7 @@ __global__ void __launch_bounds__(256) k_fbar(Ctrl *c, Geom g, PState<T> P, GSta
 #pragma unroll
             for (int j = 0; j < D; j++) dF[i][j] *= f;
     }
-    constitutive<D, T>(c, P, M, p, dF);
+    constitutive<D, T, SURR>(c, P, M, p, dF);
 }
 
 }  // namespace ep

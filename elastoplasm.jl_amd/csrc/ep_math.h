@@ -228,14 +228,17 @@ __device__ __forceinline__ void lu_solve(const DelLU &f, const T *b, T *x) {
 
 // σTr / materialParam / σn and the closed-form Drucker-Prager return
 // (DP.jl:1-62) on one Voigt stress.  Returns 0 elastic, 1 shear, 2 apex.
+// The step passes ψ = 0 as finite_DP / infinitesimal_DP hard-code it
+// (DP.jl:37, 73); the surrogate's training samples use materialParam's ψ
+// argument (DP.jl:13-24) for ψ in [0, φ].
 template <int NSTR, typename T>
-__device__ __forceinline__ int dp_return(T *s, T c0, T cr, T phi, T Hp, T Kc, T Gc, T epII2, T &epII1, T &dlam) {
+__device__ __forceinline__ int dp_return(T *s, T c0, T cr, T phi, T Hp, T Kc, T Gc, T epII2, T &epII1, T &dlam,
+                                         T psi = T(0)) {
     constexpr int ND = NSTR == 3 ? 2 : 3;
     dlam = T(0);
     T c = c0 + Hp * epII2;
     if (c < cr) c = cr;
     T P, t0[NSTR], tII, eta, etaB, xi;
-    const T psi = T(0);
     if (NSTR == 3) {
         P = (s[0] + s[1]) / T(2);
         t0[0] = s[0] - P; t0[1] = s[1] - P; t0[2] = s[2] - T(0);
@@ -281,6 +284,39 @@ __device__ __forceinline__ int dp_return(T *s, T c0, T cr, T phi, T Hp, T Kc, T 
         branch = 2;
     }
     return branch;
+}
+
+// The branch dp_return would take on s (0 elastic, 1 shear, 2 apex), with the
+// same arithmetic and ψ = 0, without changing anything.
+template <int NSTR, typename T>
+__device__ __forceinline__ int dp_trial(const T *s, T c0, T cr, T phi, T Hp, T epII2) {
+    T c = c0 + Hp * epII2;
+    if (c < cr) c = cr;
+    T P, t0[NSTR], tII, eta, xi;
+    if (NSTR == 3) {
+        P = (s[0] + s[1]) / T(2);
+        t0[0] = s[0] - P; t0[1] = s[1] - P; t0[2] = s[2] - T(0);
+        tII = sqrt(T(0.5) * (t0[0] * t0[0] + t0[1] * t0[1]) + t0[2] * t0[2]);
+        const T tp = tan(phi);
+        eta = T(3) * tp / (sqrt(T(9) + T(12) * tp * tp));
+        xi = T(3) * c / (sqrt(T(9) + T(12) * tp * tp));
+    } else {
+        P = (s[0] + s[1] + s[2]) / T(3);
+#pragma unroll
+        for (int k = 0; k < 3; k++) t0[k] = s[k] - P;
+#pragma unroll
+        for (int k = 3; k < NSTR; k++) t0[k] = s[k] - T(0);
+        tII = sqrt(T(0.5) * (t0[0] * t0[0] + t0[1] * t0[1] + t0[2] * t0[2]) + t0[3] * t0[3] + t0[4] * t0[4] + t0[5] * t0[5]);
+        const T sp = sin(phi);
+        eta = T(6) * sp / (sqrt(T(3)) * (T(3) + sp));
+        xi = T(6) * c * cos(phi) / (sqrt(T(3)) * (T(3) + sp));
+    }
+    const T sm = xi / eta, tP = xi - eta * (xi / eta);
+    const T fs = tII + eta * P - xi;
+    const T h = tII - tP - (sqrt(T(1) + eta * eta)) * (P - sm);
+    if ((fs > T(0) && P < sm) || (h > T(0) && P >= sm)) return 1;
+    if (h <= T(0) && P >= sm) return 2;
+    return 0;
 }
 
 // get_J2 / yield_J2 (J2.jl:1-20)

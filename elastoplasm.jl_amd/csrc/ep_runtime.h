@@ -7,6 +7,7 @@
 #define EP_BLOCK 64
 #define EP_LDS
 inline void lds_atomic_add(double *p, double v) { std::atomic_ref<double>(*p).fetch_add(v); }
+#define EP_MFMA_32X32X8_BF16(a, b, c) emu_mfma_32x32x8bf16(a, b, c)
 #else
 #include <hip/hip_runtime.h>
 #define EP_BLOCK 256
@@ -17,4 +18,9 @@ __device__ __forceinline__ void lds_atomic_add(EP_LDS double *p, double v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 #define EP_LAUNCH(kern, grid, block, stream, ...) hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, stream, __VA_ARGS__)
+// MFMA operand types: 4 bf16 (as raw 16-bit patterns) and 16 FP32 accumulators
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+#define EP_MFMA_32X32X8_BF16(a, b, c) __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0)
 #endif

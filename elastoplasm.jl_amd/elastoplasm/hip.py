@@ -68,7 +68,12 @@ class EpStats(C.Structure):
 
 EXPORTS = ["ep_version", "ep_last_error", "ep_create", "ep_destroy", "ep_upload_grid", "ep_upload_particles",
            "ep_download_particles", "ep_download_grid", "ep_set_time", "ep_step", "ep_get_dts", "ep_profile",
-           "ep_kernel_times", "ep_get_stats"]
+           "ep_kernel_times", "ep_get_stats", "ep_surrogate_create", "ep_surrogate_destroy",
+           "ep_surrogate_get_params", "ep_surrogate_set_params", "ep_surrogate_sample", "ep_surrogate_get_samples",
+           "ep_surrogate_train", "ep_surrogate_grad", "ep_surrogate_predict", "ep_attach_surrogate"]
+
+NPARAM = 34692
+_fp = C.POINTER(C.c_float)
 
 _lib = None
 
@@ -102,6 +107,17 @@ def load() -> C.CDLL:
     L.ep_kernel_times.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_int32, _dp,
                                   C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
     L.ep_get_stats.argtypes = [C.c_void_p, C.POINTER(EpStats)]
+    L.ep_surrogate_create.argtypes = [C.c_uint64, C.POINTER(C.c_void_p)]
+    L.ep_surrogate_destroy.argtypes = [C.c_void_p]
+    L.ep_surrogate_destroy.restype = None
+    L.ep_surrogate_get_params.argtypes = [C.c_void_p, _fp]
+    L.ep_surrogate_set_params.argtypes = [C.c_void_p, _fp]
+    L.ep_surrogate_sample.argtypes = [C.c_void_p, C.c_int64, C.c_uint64]
+    L.ep_surrogate_get_samples.argtypes = [C.c_void_p, C.c_int64, C.c_int64, _fp, _fp, _dp]
+    L.ep_surrogate_train.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_double, C.c_double, _fp]
+    L.ep_surrogate_grad.argtypes = [C.c_void_p, _fp, _fp, C.c_int64, _fp, _fp]
+    L.ep_surrogate_predict.argtypes = [C.c_void_p, _fp, C.c_int64, _fp]
+    L.ep_attach_surrogate.argtypes = [C.c_void_p, C.c_void_p, C.c_double]
     _lib = L
     return L
 
@@ -226,6 +242,11 @@ class Device:
             out[nm] = (ms[k], la[k])
         return out
 
+    def attach_surrogate(self, net: "Surrogate", tol: float = 0.0):
+        """Network for a precision EP_FP32_SURROGATE context (kept alive here)."""
+        self._net = net
+        _check(self.L.ep_attach_surrogate(self.h, net.h, float(tol)))
+
     def stats(self) -> EpStats:
         st = EpStats()
         _check(self.L.ep_get_stats(self.h, C.byref(st)))
@@ -234,6 +255,80 @@ class Device:
     def close(self):
         if getattr(self, "h", None):
             self.L.ep_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _f32(a, shape):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.shape != shape:
+        raise ValueError(f"expected shape {shape}, got {a.shape}")
+    return a
+
+
+class Surrogate:
+    """The learned Drucker-Prager return (ep_surrogate_*): an MLP
+    8-128-128-128-4 trained and evaluated on the device with bf16 MFMA."""
+
+    def __init__(self, seed: int = 0):
+        self.L = load()
+        h = C.c_void_p()
+        _check(self.L.ep_surrogate_create(int(seed), C.byref(h)))
+        self.h = h
+
+    @property
+    def params(self) -> np.ndarray:
+        out = np.zeros(NPARAM, np.float32)
+        _check(self.L.ep_surrogate_get_params(self.h, out.ctypes.data_as(_fp)))
+        return out
+
+    @params.setter
+    def params(self, w):
+        w = _f32(w, (NPARAM,))
+        _check(self.L.ep_surrogate_set_params(self.h, w.ctypes.data_as(_fp)))
+
+    def sample(self, n: int, seed: int):
+        _check(self.L.ep_surrogate_sample(self.h, int(n), int(seed)))
+
+    def samples(self, first: int, n: int):
+        x = np.zeros((n, 8), np.float32)
+        y = np.zeros((n, 4), np.float32)
+        raw = np.zeros((n, 6), np.float64)
+        _check(self.L.ep_surrogate_get_samples(self.h, int(first), int(n), x.ctypes.data_as(_fp),
+                                                y.ctypes.data_as(_fp), raw.ctypes.data_as(_dp)))
+        return x, y, raw
+
+    def train(self, steps: int, batch: int, lr0: float, lr1: float) -> np.ndarray:
+        loss = np.zeros(max(int(steps), 1), np.float32)
+        _check(self.L.ep_surrogate_train(self.h, int(steps), int(batch), float(lr0), float(lr1),
+                                         loss.ctypes.data_as(_fp)))
+        return loss[: int(steps)]
+
+    def grad(self, x, y):
+        n = len(x)
+        x = _f32(x, (n, 8))
+        y = _f32(y, (n, 4))
+        g = np.zeros(NPARAM, np.float32)
+        loss = np.zeros(1, np.float32)
+        _check(self.L.ep_surrogate_grad(self.h, x.ctypes.data_as(_fp), y.ctypes.data_as(_fp), n,
+                                        g.ctypes.data_as(_fp), loss.ctypes.data_as(_fp)))
+        return g, float(loss[0])
+
+    def predict(self, x) -> np.ndarray:
+        n = len(x)
+        x = _f32(x, (n, 8))
+        y = np.zeros((n, 4), np.float32)
+        _check(self.L.ep_surrogate_predict(self.h, x.ctypes.data_as(_fp), n, y.ctypes.data_as(_fp)))
+        return y
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.ep_surrogate_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -169,6 +169,37 @@ int ep_kernel_times(ep_ctx *ctx, int32_t max_kernels, char *names, int32_t name_
                     double *total_ms, int64_t *launches, int32_t *count);
 int ep_get_stats(ep_ctx *ctx, ep_stats *st);
 
+/* ---- learned return mapping (no counterpart in the reference) -------------
+ * A network 8-128-128-128-4 (SiLU) that predicts the returned principal
+ * Kirchhoff stresses and Δλ of the 3D Drucker-Prager return (DP.jl:1-62)
+ * from the trial principal log strains and E, ν, c, φ, ψ; bf16 MFMA with
+ * FP32 accumulation, trained with Adam on the device.  Used by contexts
+ * created with precision EP_FP32_SURROGATE: every plastic trial state is
+ * predicted, checked against the flow rule and yield condition (scaled
+ * residual <= tol) and replaced by the FP64 closed-form return otherwise. */
+#define EP_SURR_NPARAM 34692
+typedef struct ep_surrogate ep_surrogate;
+int ep_surrogate_create(uint64_t seed, ep_surrogate **out);
+void ep_surrogate_destroy(ep_surrogate *s);
+/* FP32 master parameters, [W1 (128x8), b1, W2 (128x128), b2, W3, b3,
+ * W4 (4x128), b4], W row-major [out][in]; set resets the Adam moments. */
+int ep_surrogate_get_params(ep_surrogate *s, float *params);
+int ep_surrogate_set_params(ep_surrogate *s, const float *params);
+/* Device training set of n samples: inputs x [n][8], scaled targets y [n][4],
+ * raw [n][6] = FP64 (τ'1, τ'2, τ'3, Δλ, S, branch). */
+int ep_surrogate_sample(ep_surrogate *s, int64_t n, uint64_t seed);
+int ep_surrogate_get_samples(ep_surrogate *s, int64_t first, int64_t n, float *x, float *y, double *raw);
+/* `steps` Adam steps over consecutive batches of the training set, learning
+ * rate cosine from lr0 to lr1; loss[steps] (may be NULL) = batch MSE. */
+int ep_surrogate_train(ep_surrogate *s, int64_t steps, int64_t batch, double lr0, double lr1, float *loss);
+/* Gradient of the MSE over n samples (n a multiple of 32), no update. */
+int ep_surrogate_grad(ep_surrogate *s, const float *x, const float *y, int64_t n, float *grad, float *loss);
+int ep_surrogate_predict(ep_surrogate *s, const float *x, int64_t n, float *y);
+/* Network used by an EP_FP32_SURROGATE context; tol <= 0 takes
+ * cfg.surrogate_tol, or 1e-3 when that is 0.  The network must outlive the
+ * context's steps. */
+int ep_attach_surrogate(ep_ctx *ctx, ep_surrogate *s, double tol);
+
 #ifdef __cplusplus
 }
 #endif

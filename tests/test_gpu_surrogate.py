@@ -1,0 +1,187 @@
+"""The learned Drucker-Prager return on the device (ep_surrogate_* and the
+EP_FP32_SURROGATE step), against the numpy oracle in oracle/mlp.py:
+training samples, bf16 MFMA forward pass, gradients of the training loss,
+training itself, the residual-checked return inside the 3D slump step, and
+what the trained network gains over no network."""
+from __future__ import annotations
+
+import math
+import os
+
+import numpy as np
+import pytest
+
+from cases import clone, relerr, slump
+from elastoplasm import hip
+from oracle import mlp
+
+pytestmark = pytest.mark.gpu
+
+# the CPU checking build emulates every MFMA lane by lane: keep it small there
+EMU = "cpucheck" in os.environ.get("EP_HIP_LIB", "")
+C1 = dict(rho0=2650.0, E=1.0e6, nu=0.3, c0=20.0e3, cr=20.0e3, phi0=math.radians(30.0),
+          phir=math.radians(30.0), Hp=0.0)
+
+
+def _blocks(g):
+    out, o = {}, 0
+    for name, shp in mlp.SHAPES:
+        n = int(np.prod(shp))
+        out[name] = g[o:o + n]
+        o += n
+    return out
+
+
+@pytest.fixture(scope="module")
+def trained():
+    """A network trained on the device (seed 45) on 2^21 samples (seed 44)."""
+    net = hip.Surrogate(seed=45)
+    if EMU:
+        net.sample(4096, 44)
+        loss = net.train(4, 1024, 1e-3, 1e-3)
+    else:
+        net.sample(1 << 21, 44)
+        loss = net.train(6000, 65536, 2e-3, 2e-5)
+    yield net, loss
+    net.close()
+
+
+def test_training_samples_match_oracle():
+    net = hip.Surrogate(seed=1)
+    n = 1024 if EMU else 65536
+    net.sample(n, 44)
+    x, y, raw = net.samples(0, n)
+    X, Y, R, _ = mlp.samples(n, 44)
+    for k in range(6):
+        tol = 1e-12 * np.abs(R[:, k]).max()
+        assert np.abs(raw[:, k] - R[:, k]).max() <= tol, k
+    np.testing.assert_allclose(x, X, rtol=2e-6, atol=2e-6)
+    np.testing.assert_allclose(y, Y, rtol=2e-6, atol=2e-6)
+    net.close()
+
+
+def test_forward_matches_oracle():
+    net = hip.Surrogate(seed=3)
+    w = net.params.astype(np.float64) * 2.0      # weights large enough to make SiLU nonlinear
+    net.params = w
+    X, _, _, _ = mlp.samples(256 if EMU else 8192, 7)
+    y = net.predict(X)
+    yb, _ = mlp.forward(net.params, X, bf16_ops=True)
+    y64, _ = mlp.forward(net.params, X, bf16_ops=False)
+    scale = np.abs(yb).max()
+    e_bf = np.abs(y - yb).max() / scale
+    e_64 = np.abs(y - y64).max() / scale
+    print(f"forward: max err vs bf16-emulating oracle {e_bf:.2e}, vs FP64 oracle {e_64:.2e}")
+    assert e_bf <= 2e-3
+    assert e_64 <= 5e-2
+    net.close()
+
+
+def test_gradient_matches_oracle():
+    net = hip.Surrogate(seed=4)
+    net.params = net.params.astype(np.float64) * 2.0
+    n = 512 if EMU else 4096
+    X, Y, _, _ = mlp.samples(n, 8)
+    g, loss = net.grad(X, Y)
+    lb, gb = mlp.loss_and_grad(net.params, X, Y, bf16_ops=True)
+    l64, g64 = mlp.loss_and_grad(net.params, X, Y, bf16_ops=False)
+    assert abs(loss - lb) <= 1e-3 * lb
+    G, Gb, G64 = _blocks(g), _blocks(gb), _blocks(g64)
+    for k in G:
+        eb = np.linalg.norm(G[k] - Gb[k]) / np.linalg.norm(Gb[k])
+        e64 = np.linalg.norm(G[k] - G64[k]) / np.linalg.norm(G64[k])
+        print(f"grad {k}: rel err vs bf16-emulating oracle {eb:.2e}, vs FP64 oracle {e64:.2e}")
+        assert eb <= 2e-2, k
+        assert e64 <= 1e-1, k
+    net.close()
+
+
+def test_training_lowers_loss(trained):
+    net, loss = trained
+    if EMU:
+        assert np.isfinite(loss).all()
+        return
+    first, last = loss[:10].mean(), loss[-50:].mean()
+    print(f"training loss {first:.3e} -> {last:.3e}")
+    assert last < 0.02 * first
+
+
+def _acceptance(y, aux, tol):
+    r, shear = mlp.residual(y.astype(np.float64), aux["e"], aux["E"], aux["nu"], aux["c"], aux["phi"], aux["psi"])
+    return (shear & (r <= tol)).mean(), np.median(r)
+
+
+@pytest.mark.parametrize("delta", [1.0, 1e-2])
+def test_trained_network_gain(trained, delta):
+    """Held-out regular shear returns (seed 46), from their drawn trial states
+    (delta 1) and from trial states 1/100 of the way out on the same return
+    path (the small overstress of an explicit step): the trained network's
+    predictions pass the residual check far more often than the untrained
+    network's or than keeping the elastic trial state (no network)."""
+    net, _ = trained
+    n = 256 if EMU else 65536
+    X, Y, R, aux = mlp.samples(n, 46)
+    if delta != 1.0:
+        X, aux = mlp.near_cone(aux, delta)
+    tol = 1e-3
+    a_tr, r_tr = _acceptance(net.predict(X), aux, tol)
+    fresh = hip.Surrogate(seed=99)
+    a_un, r_un = _acceptance(fresh.predict(X), aux, tol)
+    fresh.close()
+    a_no, r_no = _acceptance(np.zeros_like(Y), aux, tol)
+    print(f"overstress x{delta:g}: acceptance at tol {tol}: trained {a_tr:.3f} (median residual {r_tr:.2e}), "
+          f"untrained {a_un:.3f} ({r_un:.2e}), no network {a_no:.3f} ({r_no:.2e})")
+    if EMU:
+        return
+    assert r_tr <= 0.1 * min(r_un, r_no)
+    if delta < 1.0:
+        assert a_tr >= 0.6 and a_tr >= a_no + 0.4 and a_un <= 0.3
+
+
+WEAK = dict(C1, c0=100.0, cr=100.0)
+TOL_STEP = 1e-4     # scaled residual a prediction must meet inside the step
+
+
+@pytest.mark.parametrize("mat", ["C1", "weak"])
+def test_step_with_learned_return(trained, mat):
+    """3D slump: FP32 + learned return vs the FP64 path (C1 materials, and a
+    100 Pa cohesion that yields from the first steps)."""
+    net, _ = trained
+    if EMU and mat == "C1":
+        pytest.skip("C1 stays elastic over the few steps the CPU check can afford")
+    n = 3 if EMU else 20
+    ic, cfg = slump(3, C_pf=1.0, cmpr=C1 if mat == "C1" else WEAK)
+    d64 = clone(ic)
+    c64 = clone(cfg)
+    dev = hip.Device(d64.mesh, d64.mpts, d64.cmpr, c64.instr)
+    dev.step(n, 100.0)
+    dev.download()
+    dev.close()
+    ds = clone(ic)
+    cs = clone(cfg)
+    cs.instr["precision"] = "fp32_surrogate"
+    cs.instr["ops"]["precision"] = 2
+    dev = hip.Device(ds.mesh, ds.mpts, ds.cmpr, cs.instr)
+    dev.attach_surrogate(net, TOL_STEP)
+    r = dev.step(n, 100.0)
+    dev.download()
+    st = dev.stats()
+    dev.close()
+    assert r.steps == n
+    err = relerr(ds.mpts.s.tau, d64.mpts.s.tau)
+    x0 = ic.mpts.x
+    eu = relerr(ds.mpts.x - x0, d64.mpts.x - x0)
+    acc = st.predictions - st.fallbacks
+    print(f"learned return, {n} steps: stress err {err:.2e}, displacement err {eu:.2e}, predictions "
+          f"{st.predictions}, accepted {acc}, fallbacks {st.fallbacks}, max residual {st.max_residual:.2e}")
+    assert st.predictions > 0
+    assert 0 <= st.fallbacks <= st.predictions
+    assert err <= 1e-3 and eu <= 1e-3
+
+
+def test_surrogate_needs_3d_finite_dp():
+    ic, cfg = slump(2, C_pf=1.0, cmpr=C1)
+    cfg = clone(cfg)
+    cfg.instr["ops"]["precision"] = 2
+    with pytest.raises(hip.EpError, match="EP_E_CONFIG"):
+        hip.Device(ic.mesh, ic.mpts, ic.cmpr, cfg.instr)
