@@ -109,6 +109,42 @@ def cpu_baseline(name: str, budget_s: float, threads: int):
                       f"same column, {nmp} particles, {steps} steps in {el:.1f} s, OpenMP {threads} threads"}
 
 
+def ensemble_leg(steps: int) -> dict:
+    """256 independent 2D slumps of uneven size (nel 40x10 .. 320x80, seed
+    43), FP32, cubic B-spline, TPIC-USL: autotuned layout, then `steps` graph
+    replays (one per step of every instance)."""
+    t0 = time.perf_counter()
+    lst, draw = ep.ic_ensemble(256, 43)
+    devs = []
+    for ic, cfg in lst:
+        devs.append(hip.Device(ic.mesh, ic.mpts, ic.cmpr, cfg.instr))
+    e = hip.Ensemble(devs)
+    nmp = sum(ic.mpts.nmp for ic, _ in lst)
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    tune = e.tune(5, 1e9)
+    t_tune = time.perf_counter() - t0
+    e.step(10, 1e9)
+    t0 = time.perf_counter()
+    r = e.step(steps, 1e9)
+    el = time.perf_counter() - t0
+    assert all(x[2] == steps for x in r), "instances skipped steps"
+    e.download()
+    finite = all(bool(np.isfinite(ic.mpts.s.tau).all()) for ic, _ in lst)
+    e.close()
+    for d in devs:
+        d.close()
+    value = nmp * steps / el
+    print(f"[bench] ensemble: 256 slumps, {nmp} particles, {steps} steps in {el:.3f}s = {value:.4g} pu/s "
+          f"({1000 * el / steps:.3f} ms/step); tuned block {tune['block']} brick {tune['brick']} "
+          f"(setup {t_setup:.1f}s, tuning {t_tune:.1f}s)", file=sys.stderr, flush=True)
+    return {"instances": 256, "nmp_total": nmp, "nmp_min": int(min(ic.mpts.nmp for ic, _ in lst)),
+            "nmp_max": int(max(ic.mpts.nmp for ic, _ in lst)), "steps": steps, "value": value,
+            "unit": "particle-updates/s", "ms_per_step": 1000 * el / steps, "dtype": "f32",
+            "scheme": "cubic B-spline, TPIC-USL, finite, F-bar, Drucker-Prager", "autotune": tune,
+            "state_finite": finite}
+
+
 def measured_traffic(workload: str, precision: str, kernel: str):
     """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE /
     WRITE_SIZE passes summarised in profiles/hbm_traffic_<workload>.md
@@ -138,6 +174,7 @@ def main():
                     help="fp64 alone gives the headline line without the comparisons (used under profilers)")
     ap.add_argument("--surr-samples", type=int, default=10_000_000, help="training samples of the learned return")
     ap.add_argument("--surr-steps", type=int, default=6000, help="Adam steps (batch 65536)")
+    ap.add_argument("--ensemble-steps", type=int, default=1000, help="steps of the 256-slump ensemble (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the step would shard by spatial "
@@ -247,6 +284,9 @@ def main():
                      "max_scaled_residual": float(stt.max_residual), "tol": SURR_TOL,
                      "kernels_us": {k: v * 1000 for k, v in pks.items()}})
         net.close()
+    ens = None
+    if args.ensemble_steps > 0 and args.precision == "all":
+        ens = ensemble_leg(args.ensemble_steps)
     cpu = None
     if not args.no_cpu_baseline:
         thr = min(16, os.cpu_count() or 1)
@@ -272,6 +312,7 @@ def main():
         "kernels_us": {k: v * 1000 for k, v in per_kernel.items()},
         "fp32": fp32,
         "fp32_surrogate": surr,
+        "ensemble": ens,
         "state_finite": finite,
         "cpu_baseline": cpu,
     }

@@ -6,6 +6,7 @@
 // touches a compact box of nodes; download undoes the permutation.
 #include "../../include/elastoplasm.h"
 #include "ep_mlp.h"
+#include "ep_batch.h"
 
 #include <algorithm>
 #include <cmath>
@@ -88,6 +89,9 @@ struct ep_ctx {
     std::vector<long> perm;          // device slot -> host particle index
     std::vector<double> n0_h, rho0_h, Om0_h;
     bool part_ready = false;
+    // launch shape: threads per particle/node block and cells per sort brick
+    int tb = EP_BLOCK;
+    long brick[3] = {4, 2, 4};
     // learned return (EP_FP32_SURROGATE): queue of plastic trial states
     std::vector<void *> q_allocs;
     ep_surrogate *surr = nullptr;
@@ -139,6 +143,7 @@ extern "C" int ep_create(const ep_config *cfg, ep_ctx **out) {
     c->fp32 = cfg->precision != EP_FP64;
     c->dim = cfg->dim;
     c->nstr = cfg->dim == 2 ? 3 : 6;
+    if (c->dim == 2) { c->brick[0] = 8; c->brick[1] = 8; c->brick[2] = 1; }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return fail(EP_E_HIP, "hipStreamCreate failed");
@@ -293,6 +298,7 @@ extern "C" int ep_upload_grid(ep_ctx *c, const ep_grid_view *gv) {
 static long brick_key(const ep_ctx *c, const double *x, long p) {
     const int D = c->dim;
     const Geom &g = c->geom;
+    const long *b = c->brick;   // cells per brick along each axis
     long e[3];
     for (int i = 0; i < D; i++) {
         double q = host_exact_fld(x[i + D * p] - g.x0[i], g.h[i]);
@@ -301,13 +307,13 @@ static long brick_key(const ep_ctx *c, const double *x, long p) {
         e[i] = long(q);
     }
     if (D == 2) {
-        const long nbz = (g.nel[1] + 7) / 8;
-        const long brick = e[1] / 8 + nbz * (e[0] / 8);
-        return brick * 64 + (e[1] % 8) + 8 * (e[0] % 8);
+        const long nbz = (g.nel[1] + b[1] - 1) / b[1];
+        const long brick = e[1] / b[1] + nbz * (e[0] / b[0]);
+        return brick * (b[0] * b[1]) + (e[1] % b[1]) + b[1] * (e[0] % b[0]);
     }
-    const long nbz = (g.nel[2] + 3) / 4, nbx = (g.nel[0] + 3) / 4;
-    const long brick = e[2] / 4 + nbz * (e[0] / 4 + nbx * (e[1] / 2));
-    return brick * 32 + (e[2] % 4) + 4 * ((e[0] % 4) + 4 * (e[1] % 2));
+    const long nbz = (g.nel[2] + b[2] - 1) / b[2], nbx = (g.nel[0] + b[0] - 1) / b[0];
+    const long brick = e[2] / b[2] + nbz * (e[0] / b[0] + nbx * (e[1] / b[1]));
+    return brick * (b[0] * b[1] * b[2]) + (e[2] % b[2]) + b[2] * ((e[0] % b[0]) + b[0] * (e[1] % b[1]));
 }
 
 // Particle fields in device order, one allocation per field, stored as T.
@@ -442,7 +448,7 @@ extern "C" int ep_upload_particles(ep_ctx *c, const ep_particles_view *v) {
     h.err = 0;
     hipFree(c->vmax_part);
     c->vmax_part = nullptr;
-    h.nblk = int((n + EP_BLOCK - 1) / EP_BLOCK);
+    h.nblk = int((n + c->tb - 1) / c->tb);
     HIPCHK(hipMalloc(&c->vmax_part, sizeof(unsigned long long) * 3 * size_t(h.nblk)));
     HIPCHK(hipMemset(c->vmax_part, 0, sizeof(unsigned long long) * 3 * size_t(h.nblk)));
     h.vmax_part = c->vmax_part;
@@ -528,6 +534,14 @@ static int download_fields(ep_ctx *c, ep_particles_view *v, const PState<T> &P, 
     if (v->dlam) GET(P.dlam, v->dlam[p] = val);
     if (v->gradv && M.tpic) for (int i = 0; i < DD; i++) GET(P.gv[i], v->gradv[i + DD * p] = val);
     if (v->l && c->cfg.basis == EP_GIMPM) for (int i = 0; i < D; i++) GET(P.l0[i], v->l[i + D * p] = val);
+    if (v->l0 && c->cfg.basis == EP_GIMPM) for (int i = 0; i < D; i++) GET(P.l0[i], v->l0[i + D * p] = val);
+    if (v->c0) GET(P.c0, v->c0[p] = val);
+    if (v->cr) GET(P.cr, v->cr[p] = val);
+    if (v->phi) GET(P.phi, v->phi[p] = val);
+    if (v->rho0) for (long p = 0; p < n; p++) v->rho0[p] = c->rho0_h[p];
+    if (v->Om0) for (long p = 0; p < n; p++) v->Om0[p] = c->Om0_h[p];
+    if (v->n0) for (long p = 0; p < n; p++) v->n0[p] = c->n0_h[p];
+    if (!M.finite && v->rho) GET(P.rho, v->rho[p] = val);
 #undef GET
     return EP_OK;
 }
@@ -601,7 +615,7 @@ static mlp::SurrNet surr_net(ep_ctx *c);
 
 template <int D, int B, typename T, bool SURR>
 static void launch_step(ep_ctx *c, const PState<T> &P) {
-    const int TB = EP_BLOCK;
+    const int TB = c->tb;
     const long n = c->nmp, nno = c->gs.nno;
     const dim3 gp((n + TB - 1) / TB), gg((nno + TB - 1) / TB);
     hipStream_t s = c->stream;
@@ -1009,5 +1023,328 @@ extern "C" int ep_attach_surrogate(ep_ctx *c, ep_surrogate *s, double tol) {
     c->surr = s;
     const double t = tol > 0.0 ? tol : (c->cfg.surrogate_tol > 0.0 ? c->cfg.surrogate_tol : 1e-3);
     c->surr_tol = float(t);
+    return EP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// ensemble: many contexts advanced by one captured graph per step
+// ---------------------------------------------------------------------------
+struct ep_ensemble {
+    std::vector<ep_ctx *> ctx;
+    int n = 0, dim = 0, basis = 0;
+    bool fp32 = false;
+    hipStream_t stream = nullptr;
+    void *inst = nullptr;                  // Inst<T>[n]
+    int *pm_inst = nullptr, *pm_first = nullptr, *gm_inst = nullptr, *gm_first = nullptr;
+    long npblk = 0, ngblk = 0;
+    double *collect = nullptr;             // [n][5]
+    hipGraphExec_t exec = nullptr;
+    int tb = EP_BLOCK;
+    ep_stats stats{};
+};
+
+namespace {
+
+void ens_free_tables(ep_ensemble *e) {
+    if (e->exec) hipGraphExecDestroy(e->exec);
+    e->exec = nullptr;
+    hipFree(e->inst); hipFree(e->pm_inst); hipFree(e->pm_first); hipFree(e->gm_inst); hipFree(e->gm_first);
+    e->inst = nullptr;
+    e->pm_inst = e->pm_first = e->gm_inst = e->gm_first = nullptr;
+}
+
+// Give a context tb threads per block: its per-block vmax partials and the
+// block count in its control block follow.
+int ctx_set_block(ep_ctx *c, int tb) {
+    if (c->tb == tb && c->vmax_part) return EP_OK;
+    c->tb = tb;
+    Ctrl h{};
+    HIPCHK(hipMemcpy(&h, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+    hipFree(c->vmax_part);
+    c->vmax_part = nullptr;
+    h.nblk = int((c->nmp + tb - 1) / tb);
+    HIPCHK(hipMalloc(&c->vmax_part, sizeof(unsigned long long) * 3 * size_t(h.nblk)));
+    HIPCHK(hipMemset(c->vmax_part, 0, sizeof(unsigned long long) * 3 * size_t(h.nblk)));
+    h.vmax_part = c->vmax_part;
+    HIPCHK(hipMemcpy(c->ctrl, &h, sizeof(Ctrl), hipMemcpyHostToDevice));
+    return EP_OK;
+}
+
+template <typename T>
+int ens_tables(ep_ensemble *e) {
+    ens_free_tables(e);
+    const int n = e->n, tb = e->tb;
+    std::vector<Inst<T>> I(n);
+    std::vector<int> pf(n + 1), gf(n + 1), pi, gi;
+    long pb = 0, gb = 0;
+    for (int i = 0; i < n; i++) {
+        ep_ctx *c = e->ctx[i];
+        int r;
+        if ((r = ctx_set_block(c, tb)) != EP_OK) return r;
+        if constexpr (std::is_same<T, float>::value) I[i] = Inst<T>{c->ctrl, c->geom, c->psf, c->gs, c->mat};
+        else I[i] = Inst<T>{c->ctrl, c->geom, c->ps, c->gs, c->mat};
+        const long np = (c->nmp + tb - 1) / tb, ng = (c->gs.nno + tb - 1) / tb;
+        pf[i] = int(pb);
+        gf[i] = int(gb);
+        for (long k = 0; k < np; k++) pi.push_back(i);
+        for (long k = 0; k < ng; k++) gi.push_back(i);
+        pb += np;
+        gb += ng;
+        if (pb > 0x7fffffffL || gb > 0x7fffffffL) return fail(EP_E_ARG, "ensemble exceeds 2^31 blocks");
+    }
+    pf[n] = int(pb);
+    gf[n] = int(gb);
+    e->npblk = pb;
+    e->ngblk = gb;
+    HIPCHK(hipMalloc(&e->inst, sizeof(Inst<T>) * n));
+    HIPCHK(hipMalloc(&e->pm_inst, sizeof(int) * std::max<long>(pb, 1)));
+    HIPCHK(hipMalloc(&e->gm_inst, sizeof(int) * std::max<long>(gb, 1)));
+    HIPCHK(hipMalloc(&e->pm_first, sizeof(int) * (n + 1)));
+    HIPCHK(hipMalloc(&e->gm_first, sizeof(int) * (n + 1)));
+    HIPCHK(hipMemcpy(e->inst, I.data(), sizeof(Inst<T>) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->pm_inst, pi.data(), sizeof(int) * pb, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->gm_inst, gi.data(), sizeof(int) * gb, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->pm_first, pf.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->gm_first, gf.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+    return EP_OK;
+}
+
+// One step of every instance, captured once into a graph.  No memset or
+// copy nodes: the grid is cleared by kb_zero_grid.
+template <int D, int B, typename T>
+int ens_capture(ep_ensemble *e) {
+    int r;
+    if ((r = ens_tables<T>(e)) != EP_OK) return r;
+    const Inst<T> *I = static_cast<const Inst<T> *>(e->inst);
+    const BlockMap pm{e->pm_inst, e->pm_first}, gm{e->gm_inst, e->gm_first};
+    const Mat &M = e->ctx[0]->mat;
+    const int n = e->n, tb = e->tb;
+    hipStream_t s = e->stream;
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    EP_LAUNCH((kb_begin<D, T>), (n + 63) / 64, 64, s, I, n);
+    EP_LAUNCH((kb_zero_grid<D, T>), e->ngblk, tb, s, I, gm);
+    EP_LAUNCH((kb_p2g<D, B, T>), e->npblk, tb, s, I, pm);
+    EP_LAUNCH((kb_euler<D, T>), e->ngblk, tb, s, I, gm);
+    EP_LAUNCH((kb_g2p<D, B, T>), e->npblk, tb, s, I, pm);
+    if (M.musl) EP_LAUNCH((kb_augv<D, T>), e->ngblk, tb, s, I, gm);
+    if (M.locking) {
+        EP_LAUNCH((kb_deform<D, B, T, true>), e->npblk, tb, s, I, pm);
+        EP_LAUNCH((kb_fbar<D, B, T>), e->npblk, tb, s, I, pm);
+    } else {
+        EP_LAUNCH((kb_deform<D, B, T, false>), e->npblk, tb, s, I, pm);
+    }
+    EP_LAUNCH(kb_end<T>, n, 4 * EP_BLOCK, s, I);
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamEndCapture(s, &g));
+    const hipError_t ie = hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (ie != hipSuccess) { e->exec = nullptr; return fail(EP_E_HIP, "hipGraphInstantiate failed"); }
+    return EP_OK;
+}
+
+int ens_build(ep_ensemble *e) {
+    const bool f = e->fp32;
+    if (e->dim == 2) return f ? ens_capture<2, 0, float>(e) : ens_capture<2, 0, double>(e);
+    return f ? ens_capture<3, 0, float>(e) : ens_capture<3, 0, double>(e);
+}
+
+template <typename T>
+void ens_params(ep_ensemble *e, const StepPar &q) {
+    EP_LAUNCH(kb_params<T>, (e->n + 63) / 64, 64, e->stream, static_cast<const Inst<T> *>(e->inst), e->n, q);
+}
+template <typename T>
+void ens_collect(ep_ensemble *e) {
+    EP_LAUNCH(kb_collect<T>, (e->n + 63) / 64, 64, e->stream, static_cast<const Inst<T> *>(e->inst), e->n,
+              e->collect);
+}
+
+// nsteps graph replays; per-instance results into res (may be NULL)
+int ens_run(ep_ensemble *e, int32_t nsteps, const ep_step_params *par, ep_step_result *res, float *ms) {
+    const StepPar q{par->T, par->G, par->tg, par->dt_fixed, par->ramp, par->with_plast, par->dt_mode};
+    if (e->fp32) ens_params<float>(e, q);
+    else ens_params<double>(e, q);
+    hipEvent_t a = nullptr, b = nullptr;
+    if (ms) {
+        HIPCHK(hipEventCreate(&a));
+        HIPCHK(hipEventCreate(&b));
+        HIPCHK(hipEventRecord(a, e->stream));
+    }
+    for (int k = 0; k < nsteps; k++) HIPCHK(hipGraphLaunch(e->exec, e->stream));
+    if (ms) {
+        HIPCHK(hipEventRecord(b, e->stream));
+        HIPCHK(hipEventSynchronize(b));
+        HIPCHK(hipEventElapsedTime(ms, a, b));
+        hipEventDestroy(a);
+        hipEventDestroy(b);
+    }
+    if (e->fp32) ens_collect<float>(e);
+    else ens_collect<double>(e);
+    HIPCHK(hipGetLastError());
+    std::vector<double> out(size_t(5) * e->n);
+    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(hipMemcpy(out.data(), e->collect, sizeof(double) * out.size(), hipMemcpyDeviceToHost));
+    int code = EP_OK;
+    for (int i = 0; i < e->n; i++) {
+        const double *o = &out[size_t(5) * i];
+        const long st = long(o[2]);
+        e->ctx[i]->stats.steps += st;
+        e->stats.steps += st;
+        e->stats.j2_maxit += long(o[4]);
+        if (res) {
+            res[i].t = o[0];
+            res[i].last_dt = o[1];
+            res[i].steps = st;
+            res[i].it = 0;
+        }
+        const unsigned err = unsigned(o[3]);
+        if (code == EP_OK && (err & ERR_DOMAIN))
+            code = fail(EP_E_DOMAIN, "Invalid position in ensemble instance " + std::to_string(i));
+        else if (code == EP_OK && (err & ERR_NONFINITE))
+            code = fail(EP_E_NONFINITE, "non-finite state in ensemble instance " + std::to_string(i));
+    }
+    e->stats.launches += long(nsteps);
+    return code;
+}
+
+struct HostState {
+    std::vector<double> f[24];
+    ep_particles_view v{};
+    double t = 0.0;
+};
+
+// download an instance into host buffers laid out as the reference's arrays
+int ens_save(ep_ctx *c, HostState &h) {
+    const long n = c->nmp;
+    const int D = c->dim, DD = D * D, NS = c->nstr;
+    double **slot[] = {&h.v.x, &h.v.v, &h.v.l0, &h.v.l, &h.v.F, &h.v.b, &h.v.gradv, &h.v.eps, &h.v.sig, &h.v.tau,
+                       &h.v.epII, &h.v.rho0, &h.v.rho, &h.v.Om0, &h.v.Om, &h.v.n0, &h.v.n, &h.v.J, &h.v.dJ,
+                       &h.v.c0, &h.v.cr, &h.v.phi, &h.v.dlam, &h.v.vmax};
+    const long len[] = {D * n, D * n, D * n, D * n, DD * n, DD * n, DD * n, DD * n, NS * n, NS * n, 2 * n,
+                        n, n, n, n, n, n, n, n, n, n, n, n, 3};
+    for (int k = 0; k < 24; k++) {
+        h.f[k].assign(size_t(len[k]), 0.0);
+        *slot[k] = h.f[k].data();
+    }
+    int r;
+    if ((r = ep_download_particles(c, &h.v)) != EP_OK) return r;
+    Ctrl hc{};
+    HIPCHK(hipMemcpy(&hc, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+    h.t = hc.t;
+    return EP_OK;
+}
+
+int ens_restore(ep_ctx *c, HostState &h) {
+    int r;
+    if ((r = ep_upload_particles(c, &h.v)) != EP_OK) return r;
+    return ep_set_time(c, h.t);
+}
+
+}  // namespace
+
+extern "C" int ep_ensemble_create(ep_ctx *const *ctxs, int32_t n, ep_ensemble **out) {
+    if (!ctxs || !out || n <= 0) return fail(EP_E_ARG, "ep_ensemble_create: bad argument");
+    *out = nullptr;
+    const ep_config &c0 = ctxs[0]->cfg;
+    for (int i = 0; i < n; i++) {
+        const ep_ctx *c = ctxs[i];
+        if (!c) return fail(EP_E_ARG, "null context in the ensemble");
+        if (!c->grid_ready || !c->part_ready) return fail(EP_E_STATE, "upload every instance before ep_ensemble_create");
+        const ep_config &k = c->cfg;
+        if (k.dim != c0.dim || k.basis != c0.basis || k.transfer != c0.transfer || k.musl != c0.musl ||
+            k.deform != c0.deform || k.plast != c0.plast || k.locking != c0.locking || k.precision != c0.precision)
+            return fail(EP_E_CONFIG, "ensemble instances must share dim, basis, scheme, plasticity and precision");
+    }
+    if (c0.basis != EP_BSMPM) return fail(EP_E_CONFIG, "ensembles run the cubic B-spline basis");
+    if (c0.precision == EP_FP32_SURROGATE) return fail(EP_E_CONFIG, "ensembles run EP_FP64 or EP_FP32");
+    ep_ensemble *e = new ep_ensemble();
+    e->ctx.assign(ctxs, ctxs + n);
+    e->n = n;
+    e->dim = c0.dim;
+    e->basis = c0.basis;
+    e->fp32 = c0.precision == EP_FP32;
+    e->tb = ctxs[0]->tb;
+    int r = EP_OK;
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) r = fail(EP_E_HIP, "hipStreamCreate failed");
+    else if (hipMalloc(&e->collect, sizeof(double) * 5 * n) != hipSuccess) r = fail(EP_E_HIP, "hipMalloc failed");
+    else r = ens_build(e);
+    if (r != EP_OK) { ep_ensemble_destroy(e); return r; }
+    *out = e;
+    return EP_OK;
+}
+
+extern "C" void ep_ensemble_destroy(ep_ensemble *e) {
+    if (!e) return;
+    if (e->stream) hipStreamSynchronize(e->stream);
+    ens_free_tables(e);
+    hipFree(e->collect);
+    if (e->stream) hipStreamDestroy(e->stream);
+    delete e;
+}
+
+extern "C" int ep_ensemble_step(ep_ensemble *e, int32_t nsteps, const ep_step_params *par, ep_step_result *res) {
+    if (!e || !par) return fail(EP_E_ARG, "ep_ensemble_step: null argument");
+    if (nsteps < 0) return fail(EP_E_ARG, "nsteps must be >= 0");
+    if (par->record_dt) return fail(EP_E_ARG, "ensembles do not record dt");
+    for (ep_ctx *c : e->ctx) HIPCHK(hipStreamSynchronize(c->stream));
+    return ens_run(e, nsteps, par, res, nullptr);
+}
+
+extern "C" int ep_ensemble_tune(ep_ensemble *e, int32_t steps, const ep_step_params *par, ep_tune_result *out) {
+    if (!e || !par || !out) return fail(EP_E_ARG, "ep_ensemble_tune: null argument");
+    if (steps < 1) return fail(EP_E_ARG, "steps must be >= 1");
+    static const int blocks[] = {64, 128, 256};
+    static const long b2[][3] = {{4, 4, 1}, {8, 8, 1}, {16, 16, 1}};
+    static const long b3[][3] = {{2, 2, 2}, {4, 2, 4}, {8, 4, 8}};
+    std::vector<HostState> saved(size_t(e->n));
+    int r;
+    for (int i = 0; i < e->n; i++)
+        if ((r = ens_save(e->ctx[i], saved[i])) != EP_OK) return r;
+    *out = ep_tune_result{};
+    double best = 1e300;
+    int bi = -1;
+    int k = 0;
+    for (int ib = 0; ib < 3; ib++) {
+        if (blocks[ib] > EP_BLOCK) continue;
+        for (int jb = 0; jb < 3; jb++) {
+            const long *br = e->dim == 2 ? b2[jb] : b3[jb];
+            e->tb = blocks[ib];
+            for (int i = 0; i < e->n; i++) {
+                ep_ctx *c = e->ctx[i];
+                for (int a = 0; a < 3; a++) c->brick[a] = br[a];
+                c->tb = e->tb;
+                if ((r = ens_restore(c, saved[i])) != EP_OK) return r;
+            }
+            if ((r = ens_build(e)) != EP_OK) return r;
+            float ms = 0.f;
+            if ((r = ens_run(e, 1, par, nullptr, nullptr)) != EP_OK) return r;   // warm-up
+            if ((r = ens_run(e, steps, par, nullptr, &ms)) != EP_OK) return r;
+            const double per = double(ms) / steps;
+            if (k < 16) {
+                out->tried_block[k] = e->tb;
+                for (int a = 0; a < 3; a++) out->tried_brick[k][a] = int32_t(br[a]);
+                out->tried_ms[k] = per;
+            }
+            if (per < best) { best = per; bi = k; }
+            k++;
+        }
+    }
+    out->ntried = k;
+    out->block = out->tried_block[bi];
+    for (int a = 0; a < 3; a++) out->brick[a] = out->tried_brick[bi][a];
+    out->ms_per_step = best;
+    // restore the saved state in the chosen layout
+    e->tb = out->block;
+    for (int i = 0; i < e->n; i++) {
+        ep_ctx *c = e->ctx[i];
+        for (int a = 0; a < 3; a++) c->brick[a] = out->brick[a];
+        c->tb = e->tb;
+        if ((r = ens_restore(c, saved[i])) != EP_OK) return r;
+    }
+    return ens_build(e);
+}
+
+extern "C" int ep_ensemble_get_stats(ep_ensemble *e, ep_stats *st) {
+    if (!e || !st) return fail(EP_E_ARG, "null argument");
+    *st = e->stats;
     return EP_OK;
 }

@@ -281,8 +281,57 @@ inline EmuTeam &emu_team(unsigned n) {
     return *t;
 }
 
+// Graph capture: launches on a capturing thread are recorded and replayed
+// by hipGraphLaunch.
+struct EmuGraph {
+    std::vector<std::function<void()>> nodes;
+};
+typedef EmuGraph *hipGraph_t;
+typedef EmuGraph *hipGraphExec_t;
+typedef void *hipGraphNode_t;
+enum hipStreamCaptureMode { hipStreamCaptureModeGlobal, hipStreamCaptureModeThreadLocal, hipStreamCaptureModeRelaxed };
+inline thread_local EmuGraph *emu_capture = nullptr;
+inline hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) {
+    emu_capture = new EmuGraph();
+    return hipSuccess;
+}
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *g) {
+    *g = emu_capture;
+    emu_capture = nullptr;
+    return hipSuccess;
+}
+inline hipError_t hipGraphInstantiate(hipGraphExec_t *e, hipGraph_t g, hipGraphNode_t *, char *, size_t) {
+    *e = new EmuGraph(*g);
+    return hipSuccess;
+}
+inline hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t) {
+    for (auto &f : e->nodes) f();
+    return hipSuccess;
+}
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t e) {
+    delete e;
+    return hipSuccess;
+}
+inline hipError_t hipGraphDestroy(hipGraph_t g) {
+    delete g;
+    return hipSuccess;
+}
+inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+
+template <class... P, class... A>
+inline void emu_launch_now(void (*k)(P...), dim3 g, dim3 b, A... args);
+
 template <class... P, class... A>
 inline void emu_launch(void (*k)(P...), dim3 g, dim3 b, A... args) {
+    if (emu_capture) {
+        emu_capture->nodes.push_back([=] { emu_launch_now(k, g, b, args...); });
+        return;
+    }
+    emu_launch_now(k, g, b, args...);
+}
+
+template <class... P, class... A>
+inline void emu_launch_now(void (*k)(P...), dim3 g, dim3 b, A... args) {
     EmuTeam &team = emu_team(b.x);
     for (unsigned bx = 0; bx < g.x; bx++) {
         team.run([&, bx](unsigned t) {

@@ -426,8 +426,7 @@ __device__ __forceinline__ void stencil_base(const Geom &g, const double *xp, in
 // get_dt / get_spacetime (physics.jl:21-26, 80-90) + loop guard
 // `while T > time.t[1]` (worflow.jl:28, 74).
 template <int D>
-__global__ void k_begin(Ctrl *c, Geom g, Mat M) {
-    if (threadIdx.x != 0) return;
+__device__ __forceinline__ void b_begin(Ctrl *c, const Geom &g, const Mat &M) {
     const double t = c->t;
     c->qcount = 0;
     c->active = (c->T > t) ? 1 : 0;
@@ -453,9 +452,14 @@ __global__ void k_begin(Ctrl *c, Geom g, Mat M) {
     if (c->record && c->steps < c->dt_cap) c->dt_log[c->steps] = dt;
 }
 
+template <int D>
+__global__ void k_begin(Ctrl *c, Geom g, Mat M) {
+    if (threadIdx.x == 0) b_begin<D>(c, g, M);
+}
+
 // End of step: vmax = max over G2P's per-block maxima (the atomic max of
 // n2p.jl:37), then time.t[1] += dt (worflow.jl:38, 84).
-__global__ void k_end(Ctrl *c) {
+__device__ __forceinline__ void b_end(Ctrl *c) {
     __shared__ unsigned long long sred[16];
     if (!c->active) return;
     const int nblk = c->nblk;
@@ -480,10 +484,12 @@ __global__ void k_end(Ctrl *c) {
     c->parity ^= 1;
 }
 
+__global__ void k_end(Ctrl *c) { b_end(c); }
+
 // zero mᵢ, p, oobf (p2n.jl:290-292), the mUSL momentum and ΔJ (update.jl:53)
 template <int D>
-__global__ void k_zero_grid(const Ctrl *c, GState G) {
-    const long n = blockIdx.x * long(blockDim.x) + threadIdx.x;
+__device__ __forceinline__ void b_zero_grid(const Ctrl *c, const GState &G, long blk) {
+    const long n = blk * long(blockDim.x) + threadIdx.x;
     if (n >= G.nno || !c->active) return;
     G.m[n] = 0.0;
     G.dJ[n] = 0.0;
@@ -491,15 +497,19 @@ __global__ void k_zero_grid(const Ctrl *c, GState G) {
     for (int d = 0; d < D; d++) { G.p[d][n] = 0.0; G.f[d][n] = 0.0; G.pa[d][n] = 0.0; }
 }
 
+// kernel of one context: block b of the launch is block b of the particles / nodes
+template <int D>
+__global__ void k_zero_grid(const Ctrl *c, GState G) { b_zero_grid<D>(c, G, blockIdx.x); }
+
 // transform + std/tpic_{2,3}d_p2n (p2n.jl:12-21, 58-102, 140-191)
 template <int D, int B, typename T>
-__global__ void __launch_bounds__(256) k_p2g(const Ctrl *c, Geom g, PState<T> P, GState G, Mat M) {
+__device__ __forceinline__ void b_p2g(const Ctrl *c, const Geom &g, const PState<T> &P, const GState &G, const Mat &M, long blk) {
     constexpr int NS = D == 2 ? 3 : 6;
     constexpr int NV = 1 + 2 * D;
     __shared__ double smem[ACC_LDS];
     __shared__ int sint[8];
     if (!c->active) return;
-    const long p = blockIdx.x * long(blockDim.x) + threadIdx.x;
+    const long p = blk * long(blockDim.x) + threadIdx.x;
     const bool live = p < P.n;
     const long q = live ? p : 0;
     const int par = c->parity;
@@ -577,11 +587,15 @@ __global__ void __launch_bounds__(256) k_p2g(const Ctrl *c, Geom g, PState<T> P,
     if (lds) box.template flush<NV>(g, dst);
 }
 
+// kernel of one context: block b of the launch is block b of the particles / nodes
+template <int D, int B, typename T>
+__global__ void __launch_bounds__(256) k_p2g(const Ctrl *c, Geom g, PState<T> P, GState G, Mat M) { b_p2g<D, B, T>(c, g, P, G, M, blockIdx.x); }
+
 // euler (solve.jl:14-40); a and v of massless nodes are the zeros that
 // solve() writes first (solve.jl:58-59).
 template <int D>
-__global__ void k_euler(const Ctrl *c, GState G, Mat M) {
-    const long no = blockIdx.x * long(blockDim.x) + threadIdx.x;
+__device__ __forceinline__ void b_euler(const Ctrl *c, const GState &G, const Mat &M, long blk) {
+    const long no = blk * long(blockDim.x) + threadIdx.x;
     if (no >= G.nno || !c->active) return;
     const double dt = c->dt, eta = M.damping;
     const double m = G.m[no];
@@ -615,14 +629,18 @@ __global__ void k_euler(const Ctrl *c, GState G, Mat M) {
     for (int d = 0; d < D; d++) { G.f[d][no] = o[d]; G.a[d][no] = a[d]; G.v[d][no] = v[d]; }
 }
 
+// kernel of one context: block b of the launch is block b of the particles / nodes
+template <int D>
+__global__ void k_euler(const Ctrl *c, GState G, Mat M) { b_euler<D>(c, G, M, blockIdx.x); }
+
 // picflip_n2p + augm_momentum (n2p.jl:14-40, augm.jl:13-25)
 template <int D, int B, typename T>
-__global__ void __launch_bounds__(256) k_g2p(Ctrl *c, Geom g, PState<T> P, GState G, Mat M) {
+__device__ __forceinline__ void b_g2p(Ctrl *c, const Geom &g, const PState<T> &P, const GState &G, const Mat &M, long blk) {
     __shared__ double smem[ACC_LDS];
     __shared__ int sint[8];
     __shared__ unsigned long long sred[16];
     if (!c->active) return;
-    const long p = blockIdx.x * long(blockDim.x) + threadIdx.x;
+    const long p = blk * long(blockDim.x) + threadIdx.x;
     const bool live = p < P.n;
     const long q = live ? p : 0;
     const int par = c->parity;
@@ -674,7 +692,7 @@ __global__ void __launch_bounds__(256) k_g2p(Ctrl *c, Geom g, PState<T> P, GStat
         }
         const unsigned long long bm =
             block_max_bits(live ? (unsigned long long)__double_as_longlong(double(fabs(vn[d]))) : 0ull, sred);
-        if (threadIdx.x == 0) c->vmax_part[d * c->nblk + blockIdx.x] = bm;
+        if (threadIdx.x == 0) c->vmax_part[d * c->nblk + blk] = bm;
     }
     if (bad) atomicOr(&c->err, ERR_NONFINITE);
     if (M.musl) {
@@ -697,10 +715,14 @@ __global__ void __launch_bounds__(256) k_g2p(Ctrl *c, Geom g, PState<T> P, GStat
     }
 }
 
+// kernel of one context: block b of the launch is block b of the particles / nodes
+template <int D, int B, typename T>
+__global__ void __launch_bounds__(256) k_g2p(Ctrl *c, Geom g, PState<T> P, GState G, Mat M) { b_g2p<D, B, T>(c, g, P, G, M, blockIdx.x); }
+
 // augm_velocity (augm.jl:37-53) after mesh.v .= 0 (n2p.jl:67)
 template <int D>
-__global__ void k_augv(const Ctrl *c, GState G) {
-    const long no = blockIdx.x * long(blockDim.x) + threadIdx.x;
+__device__ __forceinline__ void b_augv(const Ctrl *c, const GState &G, long blk) {
+    const long no = blk * long(blockDim.x) + threadIdx.x;
     if (no >= G.nno || !c->active) return;
     const double m = G.m[no];
     const unsigned bc = G.bc[no];
@@ -711,6 +733,10 @@ __global__ void k_augv(const Ctrl *c, GState G) {
         G.v[d][no] = v;
     }
 }
+
+// kernel of one context: block b of the launch is block b of the particles / nodes
+template <int D>
+__global__ void k_augv(const Ctrl *c, GState G) { b_augv<D>(c, G, blockIdx.x); }
 
 // The FP32 path carries the kinematics as deviations from the identity so
 // that small strains keep their relative precision: K = ΔF - I instead of
@@ -1047,11 +1073,11 @@ __device__ __forceinline__ void deform_f(const PState<T> &P, long q, T dt, const
 //   needs ΔJ = det ΔF and ρΩ = ρ₀/J·JΩ₀ of the updated J; nothing is
 //   written back, k_fbar recomputes ΔF from the same grid velocity.
 template <int D, int B, typename T, bool LOCK, bool SURR = false>
-__global__ void __launch_bounds__(256) k_deform(Ctrl *c, Geom g, PState<T> P, GState G, Mat M) {
+__device__ __forceinline__ void b_deform(Ctrl *c, const Geom &g, const PState<T> &P, const GState &G, const Mat &M, long blk) {
     __shared__ double smem[ACC_LDS];
     __shared__ int sint[8];
     if (!c->active) return;
-    const long p = blockIdx.x * long(blockDim.x) + threadIdx.x;
+    const long p = blk * long(blockDim.x) + threadIdx.x;
     const bool live = p < P.n;
     const long q = live ? p : 0;
     const int par = c->parity;
@@ -1099,17 +1125,21 @@ __global__ void __launch_bounds__(256) k_deform(Ctrl *c, Geom g, PState<T> P, GS
     }
 }
 
+// kernel of one context: block b of the launch is block b of the particles / nodes
+template <int D, int B, typename T, bool LOCK, bool SURR = false>
+__global__ void __launch_bounds__(256) k_deform(Ctrl *c, Geom g, PState<T> P, GState G, Mat M) { b_deform<D, B, T, LOCK, SURR>(c, g, P, G, M, blockIdx.x); }
+
 // F-bar (volumetric.jl:12-35) then elast/plast.  The box stages the grid
 // velocity and ΔJ_n/m_n exactly as ΔJs writes it (0 where m_n = 0); ΔF,
 // ΔJ and F_new are recomputed from the staged velocity with the same
 // arithmetic as k_deform, F and J are written here, and ΔF is scaled by
 // (Σ N ΔJ_n/ΔJ_p)^(1/d) before the constitutive update.
 template <int D, int B, typename T, bool SURR = false>
-__global__ void __launch_bounds__(256) k_fbar(Ctrl *c, Geom g, PState<T> P, GState G, Mat M) {
+__device__ __forceinline__ void b_fbar(Ctrl *c, const Geom &g, const PState<T> &P, const GState &G, const Mat &M, long blk) {
     __shared__ double smem[ACC_LDS];
     __shared__ int sint[8];
     if (!c->active) return;
-    const long p = blockIdx.x * long(blockDim.x) + threadIdx.x;
+    const long p = blk * long(blockDim.x) + threadIdx.x;
     const bool live = p < P.n;
     const long q = live ? p : 0;
     const int par = c->parity;
@@ -1185,5 +1215,9 @@ __global__ void __launch_bounds__(256) k_fbar(Ctrl *c, Geom g, PState<T> P, GSta
     }
     constitutive<D, T, SURR>(c, P, M, p, dF);
 }
+
+// kernel of one context: block b of the launch is block b of the particles / nodes
+template <int D, int B, typename T, bool SURR = false>
+__global__ void __launch_bounds__(256) k_fbar(Ctrl *c, Geom g, PState<T> P, GState G, Mat M) { b_fbar<D, B, T, SURR>(c, g, P, G, M, blockIdx.x); }
 
 }  // namespace ep

@@ -6,10 +6,11 @@ from .config import kwargser, require
 from .ic import (NT, get_geom, get_slump, setup_cmpr, setup_mesh, setup_mpts, setup_time, get_collapse,
                  get_column, mesh_e2n)
 from .types import Mesh, Point, Solid
-from .workflow import (collapse, elastodynamic_, elastoplasm, elastoplastic_, ic_collapse, ic_slump, slump,
-                       slump_)
+from .workflow import (collapse, elastodynamic_, elastoplasm, elastoplastic_, ic_collapse, ic_ensemble, ic_slump,
+                       slump, slump_)
 from . import hip
 
 __all__ = ["kwargser", "require", "NT", "get_geom", "get_slump", "setup_cmpr", "setup_mesh", "setup_mpts",
            "setup_time", "get_collapse", "get_column", "mesh_e2n", "Mesh", "Point", "Solid", "collapse",
-           "elastodynamic_", "elastoplasm", "elastoplastic_", "ic_collapse", "ic_slump", "slump", "slump_", "hip"]
+           "elastodynamic_", "elastoplasm", "elastoplastic_", "ic_collapse", "ic_ensemble", "ic_slump", "slump", "slump_",
+           "hip"]

@@ -61,6 +61,11 @@ class EpStepResult(C.Structure):
     _fields_ = [("t", C.c_double), ("last_dt", C.c_double), ("steps", C.c_int64), ("it", C.c_int64)]
 
 
+class EpTuneResult(C.Structure):
+    _fields_ = [("block", C.c_int32), ("brick", C.c_int32 * 3), ("ms_per_step", C.c_double), ("ntried", C.c_int32),
+                ("tried_block", C.c_int32 * 16), ("tried_brick", (C.c_int32 * 3) * 16), ("tried_ms", C.c_double * 16)]
+
+
 class EpStats(C.Structure):
     _fields_ = [("launches", C.c_int64), ("steps", C.c_int64), ("fallbacks", C.c_int64),
                 ("predictions", C.c_int64), ("j2_maxit", C.c_int64), ("max_residual", C.c_double)]
@@ -70,7 +75,9 @@ EXPORTS = ["ep_version", "ep_last_error", "ep_create", "ep_destroy", "ep_upload_
            "ep_download_particles", "ep_download_grid", "ep_set_time", "ep_step", "ep_get_dts", "ep_profile",
            "ep_kernel_times", "ep_get_stats", "ep_surrogate_create", "ep_surrogate_destroy",
            "ep_surrogate_get_params", "ep_surrogate_set_params", "ep_surrogate_sample", "ep_surrogate_get_samples",
-           "ep_surrogate_train", "ep_surrogate_grad", "ep_surrogate_predict", "ep_attach_surrogate"]
+           "ep_surrogate_train", "ep_surrogate_grad", "ep_surrogate_predict", "ep_attach_surrogate",
+           "ep_ensemble_create", "ep_ensemble_destroy", "ep_ensemble_step", "ep_ensemble_tune",
+           "ep_ensemble_get_stats"]
 
 NPARAM = 34692
 _fp = C.POINTER(C.c_float)
@@ -118,6 +125,12 @@ def load() -> C.CDLL:
     L.ep_surrogate_grad.argtypes = [C.c_void_p, _fp, _fp, C.c_int64, _fp, _fp]
     L.ep_surrogate_predict.argtypes = [C.c_void_p, _fp, C.c_int64, _fp]
     L.ep_attach_surrogate.argtypes = [C.c_void_p, C.c_void_p, C.c_double]
+    L.ep_ensemble_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]
+    L.ep_ensemble_destroy.argtypes = [C.c_void_p]
+    L.ep_ensemble_destroy.restype = None
+    L.ep_ensemble_step.argtypes = [C.c_void_p, C.c_int32, C.POINTER(EpStepParams), C.POINTER(EpStepResult)]
+    L.ep_ensemble_tune.argtypes = [C.c_void_p, C.c_int32, C.POINTER(EpStepParams), C.POINTER(EpTuneResult)]
+    L.ep_ensemble_get_stats.argtypes = [C.c_void_p, C.POINTER(EpStats)]
     _lib = L
     return L
 
@@ -177,6 +190,13 @@ def particles_view(mpts: Point) -> EpParticlesView:
     return v
 
 
+def step_params(T: float, *, G: float = 9.81, tg: float = 0.0, ramp: bool = False, plast: bool = True,
+                dt: float | None = None, record_dt: bool = False) -> EpStepParams:
+    return EpStepParams(T=float(T), G=float(G), tg=float(tg), ramp=1 if ramp else 0,
+                        with_plast=1 if plast else 0, dt_mode=0 if dt is None else 1,
+                        record_dt=1 if record_dt else 0, dt_fixed=0.0 if dt is None else float(dt))
+
+
 class Device:
     """One device context: a Mesh + Point pair resident in HBM."""
 
@@ -213,9 +233,7 @@ class Device:
 
     def step(self, nsteps: int, T: float, *, G: float = 9.81, tg: float = 0.0, ramp: bool = False,
              plast: bool = True, dt: float | None = None, record_dt: bool = False):
-        p = EpStepParams(T=float(T), G=float(G), tg=float(tg), ramp=1 if ramp else 0,
-                         with_plast=1 if plast else 0, dt_mode=0 if dt is None else 1,
-                         record_dt=1 if record_dt else 0, dt_fixed=0.0 if dt is None else float(dt))
+        p = step_params(T, G=G, tg=tg, ramp=ramp, plast=plast, dt=dt, record_dt=record_dt)
         r = EpStepResult()
         _check(self.L.ep_step(self.h, int(nsteps), C.byref(p), C.byref(r)))
         return r
@@ -329,6 +347,53 @@ class Surrogate:
     def close(self):
         if getattr(self, "h", None):
             self.L.ep_surrogate_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Ensemble:
+    """Many uploaded Devices advanced together, one captured graph replay per
+    step (ep_ensemble_*).  The Devices stay usable for download."""
+
+    def __init__(self, devices):
+        self.L = load()
+        self.devices = list(devices)
+        hs = (C.c_void_p * len(self.devices))(*[d.h.value for d in self.devices])
+        h = C.c_void_p()
+        _check(self.L.ep_ensemble_create(hs, len(self.devices), C.byref(h)))
+        self.h = h
+
+    def step(self, nsteps: int, T: float, **kw):
+        p = step_params(T, **kw)
+        res = (EpStepResult * len(self.devices))()
+        _check(self.L.ep_ensemble_step(self.h, int(nsteps), C.byref(p), res))
+        return [(r.t, r.last_dt, r.steps) for r in res]
+
+    def tune(self, steps: int, T: float, **kw) -> dict:
+        p = step_params(T, **kw)
+        r = EpTuneResult()
+        _check(self.L.ep_ensemble_tune(self.h, int(steps), C.byref(p), C.byref(r)))
+        tried = [{"block": r.tried_block[k], "brick": list(r.tried_brick[k]), "ms_per_step": r.tried_ms[k]}
+                 for k in range(min(r.ntried, 16))]
+        return {"block": r.block, "brick": list(r.brick), "ms_per_step": r.ms_per_step, "tried": tried}
+
+    def stats(self) -> EpStats:
+        st = EpStats()
+        _check(self.L.ep_ensemble_get_stats(self.h, C.byref(st)))
+        return st
+
+    def download(self):
+        for d in self.devices:
+            d.download()
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.ep_ensemble_destroy(self.h)
             self.h = None
 
     def __del__(self):

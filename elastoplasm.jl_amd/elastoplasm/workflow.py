@@ -104,6 +104,39 @@ def ic_slump(L, nel, fid: str = "slump", cmpr=None, geometry: str = "wedge", col
     return NT(mesh=mesh, mpts=mpts, cmpr=cmp, time=time), NT(instr=instr, paths=NT(plot=fid), misc=misc)
 
 
+ENSEMBLE_L = [64.1584, 16.0396]
+
+
+def ic_ensemble(n: int = 256, seed: int = 43, *, scales=(1, 2, 3, 4, 5, 6, 7, 8), power: float = 2.5,
+                trsfr: str = "tpic", musl: bool = False, precision: str = "fp32", **kwargs):
+    """An ensemble of n independent 2D slumps (the ic_slump wedge, 2x2
+    particles per cell) of uneven size: instance i has nel = (40 s, 10 s)
+    with s drawn from `scales` with probability ∝ s^-power (about 0.5M
+    particles over 256 instances), cohesion c log-uniform in [5, 50] kPa and
+    friction angle φ uniform in [20°, 40°] (no softening), ρ 2650, E 1e6,
+    ν 0.3; TPIC-USL (trsfr="tpic", musl=false) by default.  Returns the list
+    of (ic, cfg) pairs and the drawn (s, c, φ) per instance."""
+    rng = np.random.default_rng(seed)
+    sc = np.asarray(scales, dtype=np.int64)
+    pr = sc.astype(np.float64) ** (-power)
+    s = rng.choice(sc, size=n, p=pr / pr.sum())
+    c = 10.0 ** rng.uniform(math.log10(5.0e3), math.log10(50.0e3), size=n)
+    phi = np.radians(rng.uniform(20.0, 40.0, size=n))
+    out = []
+    for i in range(n):
+        mat = dict(rho0=2650.0, E=1.0e6, nu=0.3, c0=float(c[i]), cr=float(c[i]), phi0=float(phi[i]),
+                   phir=float(phi[i]), Hp=0.0)
+        kw = dict(basis={"which": "bsmpm", "how": None, "ghost": False},
+                  fwrk=dict(deform="finite", trsfr=trsfr, musl=musl, C_pf=1.0, locking=True, damping=0.1),
+                  nonloc=dict(status=False, ls=0.5), plast=dict(status=True, constitutive="DP"))
+        kw.update(kwargs)
+        ic, cfg = ic_slump(ENSEMBLE_L, [40 * int(s[i]), 10 * int(s[i])], cmpr=mat, **kw)
+        cfg.instr["precision"] = precision
+        cfg.instr["ops"]["precision"] = {"fp64": 0, "fp32": 1}[precision]
+        out.append((ic, cfg))
+    return out, dict(s=s, c=c, phi=phi)
+
+
 def slump(ic, cfg, workflow: str = "elastodynamic"):
     """slump.jl:73-79: runs on deep copies."""
     out = elastoplasm(copy.deepcopy(ic), copy.deepcopy(cfg), mode=workflow)

@@ -169,6 +169,34 @@ int ep_kernel_times(ep_ctx *ctx, int32_t max_kernels, char *names, int32_t name_
                     double *total_ms, int64_t *launches, int32_t *count);
 int ep_get_stats(ep_ctx *ctx, ep_stats *st);
 
+/* ---- ensemble (no counterpart in the reference) ------------------------------
+ * Many independent contexts (e.g. slumps of different sizes and materials,
+ * each created and uploaded as usual) advanced together: one step of every
+ * instance is one replay of a captured HIP graph whose kernels cover the
+ * blocks of all instances.  Each instance keeps its own dt (get_dt rule),
+ * time window and state; ep_download_particles on an instance reads it back.
+ * All instances share dim, the B-spline basis, transfer scheme, deformation
+ * framework, plasticity and precision (EP_FP64 or EP_FP32). */
+typedef struct ep_ensemble ep_ensemble;
+typedef struct ep_tune_result {
+    int32_t block;            /* threads per particle / node block chosen */
+    int32_t brick[3];         /* cells per particle-sort brick chosen (2D: x, z) */
+    double ms_per_step;       /* of the chosen layout */
+    int32_t ntried;
+    int32_t tried_block[16];
+    int32_t tried_brick[16][3];
+    double tried_ms[16];
+} ep_tune_result;
+int ep_ensemble_create(ep_ctx *const *ctxs, int32_t n, ep_ensemble **out);
+void ep_ensemble_destroy(ep_ensemble *e);
+/* res: n entries (may be NULL); record_dt must be 0. */
+int ep_ensemble_step(ep_ensemble *e, int32_t nsteps, const ep_step_params *par, ep_step_result *res);
+/* Autotuner: times `steps` graph replays for every block size (64, 128,
+ * 256) and sort-brick shape (2D 4x4, 8x8, 16x16; 3D 2x2x2, 4x2x4, 8x4x8)
+ * from the current state, keeps the fastest and restores the state. */
+int ep_ensemble_tune(ep_ensemble *e, int32_t steps, const ep_step_params *par, ep_tune_result *out);
+int ep_ensemble_get_stats(ep_ensemble *e, ep_stats *st);
+
 /* ---- learned return mapping (no counterpart in the reference) -------------
  * A network 8-128-128-128-4 (SiLU) that predicts the returned principal
  * Kirchhoff stresses and Δλ of the 3D Drucker-Prager return (DP.jl:1-62)
