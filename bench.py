@@ -122,7 +122,7 @@ def ensemble_leg(steps: int) -> dict:
     nmp = sum(ic.mpts.nmp for ic, _ in lst)
     t_setup = time.perf_counter() - t0
     t0 = time.perf_counter()
-    tune = e.tune(5, 1e9)
+    tune = e.tune(20, 1e9)
     t_tune = time.perf_counter() - t0
     e.step(10, 1e9)
     t0 = time.perf_counter()

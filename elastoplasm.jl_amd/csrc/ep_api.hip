@@ -1316,7 +1316,7 @@ extern "C" int ep_ensemble_tune(ep_ensemble *e, int32_t steps, const ep_step_par
             }
             if ((r = ens_build(e)) != EP_OK) return r;
             float ms = 0.f;
-            if ((r = ens_run(e, 1, par, nullptr, nullptr)) != EP_OK) return r;   // warm-up
+            if ((r = ens_run(e, 3, par, nullptr, nullptr)) != EP_OK) return r;   // warm-up, graph upload
             if ((r = ens_run(e, steps, par, nullptr, &ms)) != EP_OK) return r;
             const double per = double(ms) / steps;
             if (k < 16) {
