@@ -265,6 +265,19 @@ extern "C" int ep_upload_grid(ep_ctx *c, const ep_grid_view *gv) {
     for (int i = 0; i < 3; i++) {
         g.X[i] = i < D ? c->axis_block + off[i] : c->axis_block;
         g.T[i] = i < D ? c->type_block + off[i] : c->type_block;
+        // the run of interior-type (3) nodes around the middle of the axis
+        g.t3lo[i] = 1;
+        g.t3hi[i] = 0;
+        if (i < D && c->cfg.basis == EP_BSMPM) {
+            const int8_t *t = ty.data() + off[i];
+            int lo = g.nno[i] / 2, hi = lo;
+            if (t[lo] == 3) {
+                while (lo > 0 && t[lo - 1] == 3) lo--;
+                while (hi + 1 < g.nno[i] && t[hi + 1] == 3) hi++;
+                g.t3lo[i] = lo;
+                g.t3hi[i] = hi;
+            }
+        }
     }
     // m, p[D], f[D], a[D], v[D], pa[D], dJ: one allocation each
     GState &G = c->gs;

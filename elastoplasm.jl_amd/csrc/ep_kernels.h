@@ -25,6 +25,7 @@ struct Geom {
     double x0[3];
     const double *X[3];        // per-axis node coordinates
     const int8_t *T[3];        // per-axis B-spline node type (1..4, 0 invalid)
+    int t3lo[3], t3hi[3];      // per axis, the run of type-3 nodes around the middle
     long nno_tot;
 };
 
@@ -96,6 +97,17 @@ struct W {
     T w[D][4], dw[D][4];
     int base[D];
     unsigned ok;  // bit (4*axis + k): node exists
+    // A lane past the last particle: no node, zero weights, and a base
+    // inside the block's node box so that box walks stay in bounds.
+    __device__ __forceinline__ void retire(const int *lo) {
+        ok = 0u;
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+            base[i] = lo[i];
+#pragma unroll
+            for (int k = 0; k < 4; k++) { w[i][k] = T(0); dw[i][k] = T(0); }
+        }
+    }
 };
 
 // shpfun: p2e2n + basis (tplgy.jl:43-96, bsmpm.jl:95-135, gimpm.jl:31-71,
@@ -113,6 +125,33 @@ __device__ __forceinline__ bool weights(const Geom &g, const double *xp, const T
         const int base = int(e) - 1;
         w.base[i] = base;
         const T h = T(g.h[i]), ih = T(g.ih[i]);
+        if (B == 0 && base >= g.t3lo[i] && base + 3 <= g.t3hi[i]) {
+            // Four interior nodes: when every ξ lies where bsmpm's first
+            // matching branch is the expected piece (k = 0: (1, 2], 1: (0, 1],
+            // 2: (-1, 0], 3: [-2, -1]), evaluate those pieces as bspline()
+            // does, without its branches.
+            T xi[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) xi[k] = T((xp[i] - g.X[i][base + k]) * g.ih[i]);
+            if (xi[0] > T(1) && xi[0] <= T(2) && xi[1] > T(0) && xi[1] <= T(1) && xi[2] > T(-1) && xi[2] <= T(0) &&
+                xi[3] >= T(-2) && xi[3] <= T(-1)) {
+                const T c16 = T(1.0 / 6.0), c43 = T(4.0 / 3.0), c23 = T(2.0 / 3.0);
+                T x2 = xi[0] * xi[0], x3 = x2 * xi[0];
+                w.w[i][0] = -c16 * x3 + x2 - T(2) * xi[0] + c43;
+                w.dw[i][0] = (T(-0.5) * x2 + T(2) * xi[0] - T(2)) * ih;
+                x2 = xi[1] * xi[1]; x3 = x2 * xi[1];
+                w.w[i][1] = T(0.5) * x3 - x2 + c23;
+                w.dw[i][1] = (T(1.5) * x2 - T(2) * xi[1]) * ih;
+                x2 = xi[2] * xi[2]; x3 = x2 * xi[2];
+                w.w[i][2] = T(-0.5) * x3 - x2 + c23;
+                w.dw[i][2] = (T(-1.5) * x2 - T(2) * xi[2]) * ih;
+                x2 = xi[3] * xi[3]; x3 = x2 * xi[3];
+                w.w[i][3] = c16 * x3 + x2 + T(2) * xi[3] + c43;
+                w.dw[i][3] = (T(0.5) * x2 + T(2) * xi[3] + T(2)) * ih;
+                w.ok |= 0xFu << (4 * i);
+                continue;
+            }
+        }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const int n = base + k;
@@ -214,6 +253,9 @@ __device__ __forceinline__ T mass_of(const PState<T> &P, const Mat &M, long p, T
 // the kernels fall back to global loads / global atomics.
 // -------------------------------------------------------------------------
 constexpr int ACC_LDS = 4096;   // doubles of LDS per block (32 KiB)
+// field v of a box with nv fields starts at v * lds_stride(nv): a constant, so
+// the fields of one node are immediate offsets of one LDS address
+__host__ __device__ constexpr int lds_stride(int nv) { return ACC_LDS / nv; }
 
 template <int D>
 struct BlockBox {
@@ -243,12 +285,8 @@ struct BlockBox {
         nloc = (ok && n <= ACC_LDS) ? int(n) : 0;
         __syncthreads();   // everyone read sint before a later begin() rewrites it
     }
-    __device__ __forceinline__ bool fits(int nv) const { return nloc > 0 && nloc * nv <= ACC_LDS; }
-    // local index: last axis (z) fastest, then x, then y, as the grid
-    __device__ __forceinline__ int local(const int *ix) const {
-        if (D == 2) return (ix[1] - lo[1]) + ext[1] * (ix[0] - lo[0]);
-        return (ix[2] - lo[2]) + ext[2] * ((ix[0] - lo[0]) + ext[0] * (ix[1] - lo[1]));
-    }
+    __device__ __forceinline__ bool fits(int nv) const { return nloc > 0 && nloc <= lds_stride(nv); }
+    // box entries run as the grid: last axis (z) fastest, then x, then y
     // global node of local box entry k, or -1 outside the grid
     __device__ __forceinline__ long node(const Geom &g, int k) const {
         int ix[3];
@@ -272,13 +310,15 @@ struct BlockBox {
         for (int k = threadIdx.x; k < nloc; k += blockDim.x) {
             const long no = node(g, k);
 #pragma unroll
-            for (int v = 0; v < NV; v++) buf[v * nloc + k] = no >= 0 ? src[v][no] : 0.0;
+            for (int v = 0; v < NV; v++) buf[v * lds_stride(NV) + k] = no >= 0 ? src[v][no] : 0.0;
         }
         __syncthreads();
     }
     __device__ __forceinline__ void zero(int nv) {
         __syncthreads();   // finished reading staged values
-        for (int k = threadIdx.x; k < nloc * nv; k += blockDim.x) buf[k] = 0.0;
+        const int st = lds_stride(nv);
+        for (int v = 0; v < nv; v++)
+            for (int k = threadIdx.x; k < nloc; k += blockDim.x) buf[v * st + k] = 0.0;
         __syncthreads();
     }
     template <int NV>
@@ -289,7 +329,7 @@ struct BlockBox {
             if (no < 0) continue;
 #pragma unroll
             for (int v = 0; v < NV; v++) {
-                const double val = buf[v * nloc + k];
+                const double val = buf[v * lds_stride(NV) + k];
                 if (val != 0.0) unsafeAtomicAdd(&dst[v][no], val);
             }
         }
@@ -298,33 +338,31 @@ struct BlockBox {
 
 // Scatter targets: the block's LDS box or the global nodal arrays.  Each
 // scatter loop is instantiated once per sink, so LDS traffic stays ds_*.
-template <int D>
+template <int D, int NV>
 struct LdsSink {
     BlockBox<D> *b;
-    __device__ __forceinline__ int local(const int *ix) const { return b->local(ix); }
     __device__ __forceinline__ void operator()(int v, long, int li, double val) const {
-        lds_atomic_add(&b->buf[v * b->nloc + li], val);
+        lds_atomic_add(&b->buf[v * lds_stride(NV) + li], val);
     }
 };
 struct GlobSink {
     double *const *dst;
-    __device__ __forceinline__ int local(const int *) const { return 0; }
     __device__ __forceinline__ void operator()(int v, long no, int, double val) const {
         unsafeAtomicAdd(&dst[v][no], val);
     }
 };
 // Gather sources: the staged LDS box or the global nodal arrays.
-template <int D>
+template <int D, int NV>
 struct LdsSrc {
     const BlockBox<D> *b;
-    __device__ __forceinline__ int local(const int *ix) const { return b->local(ix); }
-    __device__ __forceinline__ double operator()(int v, long, int li) const { return b->buf[v * b->nloc + li]; }
+    __device__ __forceinline__ double operator()(int v, long, int li) const { return b->buf[v * lds_stride(NV) + li]; }
 };
 struct GlobSrc {
     const double *const *src;
-    __device__ __forceinline__ int local(const int *) const { return 0; }
     __device__ __forceinline__ double operator()(int v, long no, int) const { return src[v][no]; }
 };
+
+__device__ __forceinline__ int clampi(int i, int n) { return i < 0 ? 0 : (i >= n ? n - 1 : i); }
 
 template <typename T>
 __device__ __forceinline__ T sel4(const T *a, int i) {
@@ -389,6 +427,100 @@ __device__ __forceinline__ void for_stencil_rot(const Geom &g, const W<D, T> &w,
             }
         }
     }
+}
+
+// Stencil walk over a block's LDS node box: no existence tests (nodes
+// outside the grid have zero weights, and their box slots are never
+// flushed), the box index of the particle's first node computed once, and
+// for scatters (ROT) the lane rotation applied to the weight arrays once per
+// particle instead of per node.  Without ROT the order is for_stencil's
+// reference order, and the skipped nodes of for_stencil add exact zeros here.
+template <int D, typename T, bool ROT, typename Fn>
+__device__ __forceinline__ void for_stencil_box(const BlockBox<D> &b, const W<D, T> &w, Fn &&fn) {
+    // Scatter rotation (as for_stencil_rot): z by lane % 4 and, in 3D, x by
+    // (lane / 4) % 2, so the particles of one cell in consecutive lanes hit
+    // different nodes at each step.  Of the rotations measured this one was
+    // fastest: the LDS serves 16 lanes per pass and this leaves at most two
+    // of them on one bank pair.
+    const int lane = threadIdx.x & 63;
+    const int rz = ROT ? (lane & 3) : 0;
+    T wz[4], dwz[4];
+    int oz[4];
+#pragma unroll
+    for (int j0 = 0; j0 < 4; j0++) {
+        const int j = (j0 + rz) & 3;
+        oz[j0] = j;
+        wz[j0] = ROT ? sel4(w.w[D - 1], j) : w.w[D - 1][j0];
+        dwz[j0] = ROT ? sel4(w.dw[D - 1], j) : w.dw[D - 1][j0];
+    }
+    if (D == 2) {
+        const int l0 = (w.base[1] - b.lo[1]) + b.ext[1] * (w.base[0] - b.lo[0]);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int row = l0 + b.ext[1] * i;
+#pragma unroll
+            for (int j0 = 0; j0 < 4; j0++) {
+                T dN[D];
+                const T N = w.w[0][i] * wz[j0];
+                dN[0] = w.dw[0][i] * wz[j0];
+                dN[1] = w.w[0][i] * dwz[j0];
+                const int ix[2] = {w.base[0] + i, w.base[1] + oz[j0]};
+                fn(0L, row + oz[j0], N, dN, ix);
+            }
+        }
+    } else {
+        const int rx = ROT ? ((lane >> 2) & 1) : 0;
+        T wx[4], dwx[4];
+        int ox[4];
+#pragma unroll
+        for (int i0 = 0; i0 < 4; i0++) {
+            const int i = (i0 + rx) & 3;
+            ox[i0] = i;
+            wx[i0] = ROT ? sel4(w.w[0], i) : w.w[0][i0];
+            dwx[i0] = ROT ? sel4(w.dw[0], i) : w.dw[0][i0];
+        }
+        const int l0 = (w.base[2] - b.lo[2]) + b.ext[2] * ((w.base[0] - b.lo[0]) + b.ext[0] * (w.base[1] - b.lo[1]));
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+#pragma unroll
+            for (int i0 = 0; i0 < 4; i0++) {
+                const int row = l0 + b.ext[2] * (ox[i0] + b.ext[0] * k);
+                const T wxy = wx[i0] * w.w[1][k];
+                const T dxy = dwx[i0] * w.w[1][k];
+                const T xdy = wx[i0] * w.dw[1][k];
+#pragma unroll
+                for (int j0 = 0; j0 < 4; j0++) {
+                    T dN[D];
+                    const T N = wxy * wz[j0];
+                    dN[0] = dxy * wz[j0];
+                    dN[1] = xdy * wz[j0];
+                    dN[2] = wxy * dwz[j0];
+                    const int ix[3] = {w.base[0] + ox[i0], w.base[1] + k, w.base[2] + oz[j0]};
+                    fn(0L, row + oz[j0], N, dN, ix);
+                }
+            }
+        }
+    }
+}
+
+// Walks for the two sinks / sources; fn(node, box index, N, dN, axis
+// indices).  On the box walks the axis indices may lie outside the grid
+// (zero weight): clamp them before reading node coordinates.
+template <int D, typename T, int NV, typename Fn>
+__device__ __forceinline__ void scatter_walk(const Geom &, const W<D, T> &w, const LdsSink<D, NV> &s, Fn &&fn) {
+    for_stencil_box<D, T, true>(*s.b, w, fn);
+}
+template <int D, typename T, typename Fn>
+__device__ __forceinline__ void scatter_walk(const Geom &g, const W<D, T> &w, const GlobSink &, Fn &&fn) {
+    for_stencil_rot<D, T>(g, w, [&](long no, T N, const T *dN, const int *ix) { fn(no, 0, N, dN, ix); });
+}
+template <int D, typename T, int NV, typename Fn>
+__device__ __forceinline__ void gather_walk(const Geom &, const W<D, T> &w, const LdsSrc<D, NV> &s, Fn &&fn) {
+    for_stencil_box<D, T, false>(*s.b, w, fn);
+}
+template <int D, typename T, typename Fn>
+__device__ __forceinline__ void gather_walk(const Geom &g, const W<D, T> &w, const GlobSrc &, Fn &&fn) {
+    for_stencil<D, T>(g, w, [&](long no, T N, const T *dN, const int *ix) { fn(no, 0, N, dN, ix); });
 }
 
 // Block max of non-negative doubles (as ordered bit patterns): wavefront
@@ -519,9 +651,9 @@ __device__ __forceinline__ void b_p2g(const Ctrl *c, const Geom &g, const PState
     for (int i = 0; i < D; i++) { xp[i] = P.x[par][i][q]; lp[i] = B == 1 ? P.l0[i][q] : T(0); }
     W<D, T> w;
     if (!weights<D, B, T>(g, xp, lp, w) && live) atomicOr(const_cast<unsigned *>(&c->err), ERR_DOMAIN);
-    if (!live) w.ok = 0u;
     BlockBox<D> box;
     box.begin((EP_LDS double *)smem, sint, live, w.base);
+    if (!live) w.retire(box.lo);
     const bool lds = box.fits(NV);
     if (lds) box.zero(NV);
     double *dst[NV];
@@ -546,14 +678,13 @@ __device__ __forceinline__ void b_p2g(const Ctrl *c, const Geom &g, const PState
     const T gz = T(c->g[D - 1]);
     const bool tpic = M.tpic;
     auto scatter = [&](const auto &sink) {
-    for_stencil_rot<D, T>(g, w, [&](long no, T N, const T *dN, const int *ix) {
-        const int li = sink.local(ix);
+    scatter_walk<D, T>(g, w, sink, [&](long no, int li, T N, const T *dN, const int *ix) {
         sink(0, no, li, double(N * ms));
         if (D == 2) {
             if (tpic) {
                 // Δnp = x_n - x_p (misc.jl:1-14)
-                const T dx = T(g.X[0][ix[0]] - xp[0]);
-                const T dy = T(g.X[1][ix[1]] - xp[1]);
+                const T dx = T(g.X[0][clampi(ix[0], g.nno[0])] - xp[0]);
+                const T dy = T(g.X[1][clampi(ix[1], g.nno[1])] - xp[1]);
                 sink(1, no, li, double(N * ms * (v[0] + gv[0][0] * dx + gv[0][1] * dy)));
                 sink(2, no, li, double(N * ms * (v[1] + gv[1][0] * dx + gv[1][1] * dy)));
             } else {
@@ -564,9 +695,9 @@ __device__ __forceinline__ void b_p2g(const Ctrl *c, const Geom &g, const PState
             sink(4, no, li, double(-(Om * (dN[0] * s[2] + dN[1] * s[1]) - N * (ms * gz))));
         } else {
             if (tpic) {
-                const T dx = T(g.X[0][ix[0]] - xp[0]);
-                const T dy = T(g.X[1][ix[1]] - xp[1]);
-                const T dz = T(g.X[2][ix[2]] - xp[2]);
+                const T dx = T(g.X[0][clampi(ix[0], g.nno[0])] - xp[0]);
+                const T dy = T(g.X[1][clampi(ix[1], g.nno[1])] - xp[1]);
+                const T dz = T(g.X[2][clampi(ix[2], g.nno[2])] - xp[2]);
                 sink(1, no, li, double(N * ms * (v[0] + gv[0][0] * dx + gv[0][1] * dy + gv[0][2] * dz)));
                 sink(2, no, li, double(N * ms * (v[1] + gv[1][0] * dx + gv[1][1] * dy + gv[1][2] * dz)));
                 sink(3, no, li, double(N * ms * (v[2] + gv[2][0] * dx + gv[2][1] * dy + gv[2][2] * dz)));
@@ -582,7 +713,7 @@ __device__ __forceinline__ void b_p2g(const Ctrl *c, const Geom &g, const PState
         }
     });
     };
-    if (lds) scatter(LdsSink<D>{&box});
+    if (lds) scatter(LdsSink<D, NV>{&box});
     else scatter(GlobSink{dst});
     if (lds) box.template flush<NV>(g, dst);
 }
@@ -650,9 +781,9 @@ __device__ __forceinline__ void b_g2p(Ctrl *c, const Geom &g, const PState<T> &P
     for (int i = 0; i < D; i++) { xp[i] = P.x[par][i][q]; lp[i] = B == 1 ? P.l0[i][q] : T(0); }
     W<D, T> w;
     weights<D, B, T>(g, xp, lp, w);
-    if (!live) w.ok = 0u;
     BlockBox<D> box;
     box.begin((EP_LDS double *)smem, sint, live, w.base);
+    if (!live) w.retire(box.lo);
     const bool staged = box.fits(2 * D);
     if (staged) {
         const double *src[2 * D];
@@ -667,8 +798,7 @@ __device__ __forceinline__ void b_g2p(Ctrl *c, const Geom &g, const PState<T> &P
 #pragma unroll
     for (int d = 0; d < D; d++) { gsrc[d] = G.v[d]; gsrc[D + d] = G.a[d]; }
     auto gather = [&](const auto &src) {
-        for_stencil<D, T>(g, w, [&](long no, T N, const T *, const int *ix) {
-            const int li = src.local(ix);
+        gather_walk<D, T>(g, w, src, [&](long no, int li, T N, const T *, const int *) {
 #pragma unroll
             for (int d = 0; d < D; d++) {
                 vpic[d] += N * T(src(d, no, li));
@@ -676,7 +806,7 @@ __device__ __forceinline__ void b_g2p(Ctrl *c, const Geom &g, const PState<T> &P
             }
         });
     };
-    if (staged) gather(LdsSrc<D>{&box});
+    if (staged) gather(LdsSrc<D, 2 * D>{&box});
     else gather(GlobSrc{gsrc});
     const T dt = T(c->dt), C = T(M.C_pf);
     T vn[D];
@@ -703,13 +833,12 @@ __device__ __forceinline__ void b_g2p(Ctrl *c, const Geom &g, const PState<T> &P
         for (int d = 0; d < D; d++) dst[d] = G.pa[d];
         const T ms = mass_of(P, M, q, P.J[q]);
         auto scatter = [&](const auto &sink) {
-            for_stencil_rot<D, T>(g, w, [&](long no, T N, const T *, const int *ix) {
-                const int li = sink.local(ix);
+            scatter_walk<D, T>(g, w, sink, [&](long no, int li, T N, const T *, const int *) {
 #pragma unroll
                 for (int d = 0; d < D; d++) sink(d, no, li, double(N * (ms * vn[d])));
             });
         };
-        if (lds) scatter(LdsSink<D>{&box});
+        if (lds) scatter(LdsSink<D, D>{&box});
         else scatter(GlobSink{dst});
         if (lds) box.template flush<D>(g, dst);
     }
@@ -1030,8 +1159,7 @@ __device__ __forceinline__ void grad_v(const Geom &g, const W<D, T> &w, const Sr
     for (int i = 0; i < D; i++)
 #pragma unroll
         for (int j = 0; j < D; j++) gv[i][j] = T(0);
-    for_stencil<D, T>(g, w, [&](long no, T, const T *dN, const int *ix) {
-        const int li = src.local(ix);
+    gather_walk<D, T>(g, w, src, [&](long no, int li, T, const T *dN, const int *) {
         T vn[D];
 #pragma unroll
         for (int i = 0; i < D; i++) vn[i] = T(src(v0 + i, no, li));
@@ -1087,13 +1215,13 @@ __device__ __forceinline__ void b_deform(Ctrl *c, const Geom &g, const PState<T>
     for (int i = 0; i < D; i++) { xp[i] = P.x[par][i][q]; lp[i] = B == 1 ? P.l0[i][q] : T(0); }
     W<D, T> w;
     weights<D, B, T>(g, xp, lp, w);
-    if (!live) w.ok = 0u;
     BlockBox<D> box;
     box.begin((EP_LDS double *)smem, sint, live, w.base);
+    if (!live) w.retire(box.lo);
     const bool staged = box.fits(D);
     if (staged) box.template stage<D>(g, G.v);
     T gv[D][D];
-    if (staged) grad_v<D, T>(g, w, LdsSrc<D>{&box}, 0, gv);
+    if (staged) grad_v<D, T>(g, w, LdsSrc<D, D>{&box}, 0, gv);
     else grad_v<D, T>(g, w, GlobSrc{G.v}, 0, gv);
     T dF[D][D], Fn[D][D];
     deform_f<D, T>(P, q, T(c->dt), gv, dF, Fn);
@@ -1105,11 +1233,11 @@ __device__ __forceinline__ void b_deform(Ctrl *c, const Geom &g, const PState<T>
         double *dst[1] = {G.dJ};
         const T qv = mass_of(P, M, q, J) * dJ;
         auto scatter = [&](const auto &sink) {
-            for_stencil_rot<D, T>(g, w, [&](long no, T N, const T *, const int *ix) {
-                sink(0, no, sink.local(ix), double(N * qv));
+            scatter_walk<D, T>(g, w, sink, [&](long no, int li, T N, const T *, const int *) {
+                sink(0, no, li, double(N * qv));
             });
         };
-        if (lds) scatter(LdsSink<D>{&box});
+        if (lds) scatter(LdsSink<D, 1>{&box});
         else scatter(GlobSink{dst});
         if (lds) box.template flush<1>(g, dst);
     } else if (live) {
@@ -1149,9 +1277,9 @@ __device__ __forceinline__ void b_fbar(Ctrl *c, const Geom &g, const PState<T> &
     for (int i = 0; i < D; i++) { xp[i] = P.x[par][i][q]; lp[i] = B == 1 ? P.l0[i][q] : T(0); }
     W<D, T> w;
     weights<D, B, T>(g, xp, lp, w);
-    if (!live) w.ok = 0u;
     BlockBox<D> box;
     box.begin((EP_LDS double *)smem, sint, live, w.base);
+    if (!live) w.retire(box.lo);
     const bool staged = box.fits(D + 1);
     if (staged) {
         for (int k = threadIdx.x; k < box.nloc; k += blockDim.x) {
@@ -1162,14 +1290,14 @@ __device__ __forceinline__ void b_fbar(Ctrl *c, const Geom &g, const PState<T> &
                 qv = m == 0.0 ? 0.0 : G.dJ[no] / m;
             }
 #pragma unroll
-            for (int d = 0; d < D; d++) box.buf[d * box.nloc + k] = no >= 0 ? G.v[d][no] : 0.0;
-            box.buf[D * box.nloc + k] = qv;
+            for (int d = 0; d < D; d++) box.buf[d * lds_stride(D + 1) + k] = no >= 0 ? G.v[d][no] : 0.0;
+            box.buf[D * lds_stride(D + 1) + k] = qv;
         }
         __syncthreads();
     }
     if (!live) return;
     T gv[D][D];
-    if (staged) grad_v<D, T>(g, w, LdsSrc<D>{&box}, 0, gv);
+    if (staged) grad_v<D, T>(g, w, LdsSrc<D, D + 1>{&box}, 0, gv);
     else grad_v<D, T>(g, w, GlobSrc{G.v}, 0, gv);
     T dF[D][D], Fn[D][D];
     deform_f<D, T>(P, p, T(c->dt), gv, dF, Fn);
@@ -1189,8 +1317,8 @@ __device__ __forceinline__ void b_fbar(Ctrl *c, const Geom &g, const PState<T> &
     // ΔJ̄/ΔJ_p - 1 = (Σ N δJ_n - δJ_p)/(1 + δJ_p) keeps its relative precision.
     const T rdiv = kDev<T> ? T(1) : dJp;
     if (staged) {
-        for_stencil<D, T>(g, w, [&](long, T N, const T *, const int *ix) {
-            acc += N * T(box.buf[D * box.nloc + box.local(ix)]) / rdiv;
+        gather_walk<D, T>(g, w, LdsSrc<D, D + 1>{&box}, [&](long, int li, T N, const T *, const int *) {
+            acc += N * T(box.buf[D * lds_stride(D + 1) + li]) / rdiv;
         });
     } else {
         for_stencil<D, T>(g, w, [&](long no, T N, const T *, const int *) {
