@@ -1361,3 +1361,58 @@ extern "C" int ep_ensemble_get_stats(ep_ensemble *e, ep_stats *st) {
     *st = e->stats;
     return EP_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Gaussian random field (spectral randomisation, the method of GRFS_gauss,
+// GRF.jl:66-143): g(x) = Σ_h (a_h sin(V_h·x) + b_h cos(V_h·x)) / sqrt(nh)
+// at every particle, one thread per particle, harmonics staged in LDS.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int GRF_TILE = 256;   // harmonics per LDS stage
+__global__ void k_grf(int dim, long n, const double *x, int nh, const double *waves, const double *ab, double *g) {
+    __shared__ double sv[GRF_TILE * 3], sab[GRF_TILE * 2];
+    const long p = blockIdx.x * long(blockDim.x) + threadIdx.x;
+    double xp[3] = {0.0, 0.0, 0.0};
+    if (p < n)
+        for (int i = 0; i < dim; i++) xp[i] = x[i + dim * p];
+    double acc = 0.0;
+    for (int h0 = 0; h0 < nh; h0 += GRF_TILE) {
+        const int m = nh - h0 < GRF_TILE ? nh - h0 : GRF_TILE;
+        __syncthreads();
+        for (int k = threadIdx.x; k < m; k += blockDim.x) {
+            for (int i = 0; i < 3; i++) sv[3 * k + i] = i < dim ? waves[dim * (h0 + k) + i] : 0.0;
+            sab[2 * k] = ab[2 * (h0 + k)];
+            sab[2 * k + 1] = ab[2 * (h0 + k) + 1];
+        }
+        __syncthreads();
+        for (int k = 0; k < m; k++) {
+            const double ph = sv[3 * k] * xp[0] + sv[3 * k + 1] * xp[1] + sv[3 * k + 2] * xp[2];
+            double s, c;
+            sincos(ph, &s, &c);
+            acc += sab[2 * k] * s + sab[2 * k + 1] * c;
+        }
+    }
+    if (p < n) g[p] = acc / sqrt(double(nh));
+}
+}  // namespace
+
+extern "C" int ep_grf_gauss(int32_t dim, int64_t n, const double *x, int32_t nh, const double *waves,
+                            const double *ab, double *g) {
+    if (!x || !waves || !ab || !g || n < 0 || nh <= 0 || dim < 1 || dim > 3)
+        return fail(EP_E_ARG, "ep_grf_gauss: bad argument");
+    if (n == 0) return EP_OK;
+    double *dx = nullptr, *dw = nullptr, *dab = nullptr, *dg = nullptr;
+    auto done = [&](int r) { hipFree(dx); hipFree(dw); hipFree(dab); hipFree(dg); return r; };
+    if (hipMalloc(&dx, sizeof(double) * dim * n) != hipSuccess || hipMalloc(&dw, sizeof(double) * dim * nh) != hipSuccess ||
+        hipMalloc(&dab, sizeof(double) * 2 * nh) != hipSuccess || hipMalloc(&dg, sizeof(double) * n) != hipSuccess)
+        return done(fail(EP_E_HIP, "hipMalloc failed"));
+    if (hipMemcpy(dx, x, sizeof(double) * dim * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dw, waves, sizeof(double) * dim * nh, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dab, ab, sizeof(double) * 2 * nh, hipMemcpyHostToDevice) != hipSuccess)
+        return done(fail(EP_E_HIP, "hipMemcpy failed"));
+    EP_LAUNCH(k_grf, (n + EP_BLOCK - 1) / EP_BLOCK, EP_BLOCK, 0, int(dim), long(n), dx, int(nh), dw, dab, dg);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(g, dg, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)
+        return done(fail(EP_E_HIP, "random field evaluation failed"));
+    return done(EP_OK);
+}

@@ -1231,7 +1231,14 @@ __device__ __forceinline__ void b_deform(Ctrl *c, const Geom &g, const PState<T>
         const bool lds = box.fits(1);
         if (lds) box.zero(1);
         double *dst[1] = {G.dJ};
-        const T qv = mass_of(P, M, q, J) * dJ;
+        // FP64: m_p ΔJ_p with m_p = ρΩ of the updated J (volumetric.jl:1-11).
+        // FP32: the node then holds ΔJ_n - 1 = Σ N (m_p ΔJ_p - m_p,old) / m_n,
+        // m_n = Σ N m_p,old from P2G; m_p = m_p,old for finite deformation
+        // (ρ₀/J·JΩ₀), m_p = m_p,old (1 + δJ) for infinitesimal (ρ·JΩ₀), so
+        // the scattered quantity is m δJ or m_old δJ (2 + δJ).
+        T qv;
+        if (!kDev<T> || M.finite) qv = mass_of(P, M, q, J) * dJ;
+        else qv = mass_of(P, M, q, P.J[q]) * dJ * (T(2) + dJ);
         auto scatter = [&](const auto &sink) {
             scatter_walk<D, T>(g, w, sink, [&](long no, int li, T N, const T *, const int *) {
                 sink(0, no, li, double(N * qv));

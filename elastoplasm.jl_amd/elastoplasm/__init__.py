@@ -8,9 +8,9 @@ from .ic import (NT, get_geom, get_slump, setup_cmpr, setup_mesh, setup_mpts, se
 from .types import Mesh, Point, Solid
 from .workflow import (collapse, elastodynamic_, elastoplasm, elastoplastic_, ic_collapse, ic_ensemble, ic_slump,
                        slump, slump_)
-from . import hip
+from . import grf, hip
 
 __all__ = ["kwargser", "require", "NT", "get_geom", "get_slump", "setup_cmpr", "setup_mesh", "setup_mpts",
            "setup_time", "get_collapse", "get_column", "mesh_e2n", "Mesh", "Point", "Solid", "collapse",
            "elastodynamic_", "elastoplasm", "elastoplastic_", "ic_collapse", "ic_ensemble", "ic_slump", "slump", "slump_",
-           "hip"]
+           "hip", "grf"]

@@ -169,6 +169,14 @@ int ep_kernel_times(ep_ctx *ctx, int32_t max_kernels, char *names, int32_t name_
                     double *total_ms, int64_t *launches, int32_t *count);
 int ep_get_stats(ep_ctx *ctx, ep_stats *st);
 
+/* Gaussian random field at n points by spectral randomisation, the method of
+ * GRFS_gauss (src/home/program/utils/setup/mpts/GRF.jl:66-143):
+ * g[p] = Σ_h (ab[2h] sin(V_h·x_p) + ab[2h+1] cos(V_h·x_p)) / sqrt(nh), with
+ * x [dim, n] column-major and wave vectors waves [nh][dim].  The caller draws
+ * the harmonics (seeded), so the field is reproducible. */
+int ep_grf_gauss(int32_t dim, int64_t n, const double *x, int32_t nh, const double *waves, const double *ab,
+                 double *g);
+
 /* ---- ensemble (no counterpart in the reference) ------------------------------
  * Many independent contexts (e.g. slumps of different sizes and materials,
  * each created and uploaded as usual) advanced together: one step of every

@@ -20,11 +20,11 @@ def basis_kw(which: str) -> dict:
 
 
 def slump(dim=2, *, deform="finite", trsfr="std", musl=True, locking=True, C_pf=0.99, basis="bsmpm",
-          plast="DP", nel=None, cmpr=None, geometry="wedge"):
+          plast="DP", nel=None, cmpr=None, geometry="wedge", damping=0.1):
     L = L2 if dim == 2 else L3
     nel = nel or ([40, 10] if dim == 2 else [40, 10, 10])
     kw = dict(basis=basis_kw(basis),
-              fwrk=dict(deform=deform, trsfr=trsfr, musl=musl, C_pf=C_pf, locking=locking, damping=0.1),
+              fwrk=dict(deform=deform, trsfr=trsfr, musl=musl, C_pf=C_pf, locking=locking, damping=damping),
               nonloc=dict(status=False, ls=0.5), plast=dict(status=True, constitutive=plast))
     return ep.ic_slump(L, nel, cmpr=cmpr, geometry=geometry, **kw)
 

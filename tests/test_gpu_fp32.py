@@ -48,3 +48,33 @@ def test_fp32_stress_within_1e3_of_fp64(dim, n):
     assert e64 <= 1e-6
     assert e32 <= TOL and eu <= TOL
     assert (ref.mpts.s.epII[0] > 0).sum() > 0
+
+
+J2MAT = dict(C1, c0=50.0e3, cr=50.0e3, Hp=1.0e4)
+
+
+@pytest.mark.parametrize("dim,deform,plast", [(2, "infinitesimal", "DP"), (3, "infinitesimal", "DP"),
+                                              (3, "infinitesimal", "J2"), (3, "finite", "J2"),
+                                              (2, "finite", "J2")])
+def test_fp32_frameworks_and_returns(dim, deform, plast):
+    """Jaumann-rate (infinitesimal) and J2 paths of the FP32 step, F-bar on.
+
+    The 2D infinitesimal case runs without grid damping: the damping of
+    solve.jl:20-31 switches on where |p/m| >= 1e-3, and in that run a
+    node's horizontal velocity crosses the threshold within FP32 rounding
+    of it at step 9, which moves the surface particles by O(1e-2) in one
+    step (with damping 0.1 the FP32/FP64 difference jumps from 1.3e-7 to
+    1.4e-4 there; without damping it stays 4e-7 after 30 steps)."""
+    n = 30 if dim == 2 else 10
+    damping = 0.0 if (dim == 2 and deform == "infinitesimal") else 0.1
+    ic, cfg = slump(dim, C_pf=1.0, cmpr=C1 if plast == "DP" else J2MAT, deform=deform, plast=plast,
+                    damping=damping)
+    ref = clone(ic)
+    Oracle(ref.mesh, ref.mpts, ref.cmpr, cfg.instr).run(n, 100.0)
+    d32 = run(ic, cfg, n, "fp32")
+    f = "tau" if deform == "finite" else "sig"
+    e = relerr(getattr(d32.mpts.s, f), getattr(ref.mpts.s, f))
+    x0 = ic.mpts.x
+    eu = relerr(d32.mpts.x - x0, ref.mpts.x - x0)
+    print(f"{dim}D {deform} {plast} {n} steps: fp32 stress err {e:.2e}, displacement err {eu:.2e}")
+    assert e <= TOL and eu <= TOL
