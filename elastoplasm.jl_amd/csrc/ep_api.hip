@@ -85,8 +85,10 @@ struct ep_ctx {
     PState<float> psf{};               // FP32 particle state (EP_FP32, EP_FP32_SURROGATE)
     bool fp32 = false;
     std::vector<void *> part_allocs;   // one allocation per particle field
-    long nmp = 0;
-    std::vector<long> perm;          // device slot -> host particle index
+    long nmp = 0;                      // particles
+    long nslot = 0;                    // device slots: particles + padding
+    std::vector<long> perm;            // device slot -> host particle index, -1 for padding
+    std::vector<long> donor;           // device slot -> a particle of the same cell (for padding)
     std::vector<double> n0_h, rho0_h, Om0_h;
     bool part_ready = false;
     // launch shape: threads per particle/node block and cells per sort brick
@@ -333,12 +335,11 @@ static long brick_key(const ep_ctx *c, const double *x, long p) {
 template <typename T>
 static int upload_fields(ep_ctx *c, const ep_particles_view *v, PState<T> &P) {
     const int D = c->dim, NS = c->nstr, DD = D * D;
-    const long n = c->cfg.nmp;
+    const long n = c->nslot;
     const Mat &M = c->mat;
     // one device allocation per particle field
     for (void *a : c->part_allocs) hipFree(a);
     c->part_allocs.clear();
-    c->nmp = n;
     P = PState<T>{};
     bool oom = false;
     auto take = [&]() -> T * {
@@ -364,19 +365,26 @@ static int upload_fields(ep_ctx *c, const ep_particles_view *v, PState<T> &P) {
     if (M.tpic) for (int i = 0; i < DD; i++) P.gv[i] = take();
     if (oom) return fail(EP_E_HIP, "hipMalloc of particle fields failed");
     P.n = n;
-    // host staging in device order
-    auto put = [&](auto *dst, auto getter) -> int {
+    // host staging in device order.  A padding slot copies a particle of its
+    // cell (position, material constants) unless the field has a value for
+    // padding (dead < 1e300); its volume Ω₀ = 0 marks it for the kernels.
+    auto put = [&](auto *dst, auto getter, double dead) -> int {
         using E = std::remove_pointer_t<decltype(dst)>;
         std::vector<E> st(n);
-        for (long q = 0; q < n; q++) st[q] = E(getter(c->perm[q]));
+        for (long q = 0; q < n; q++) {
+            const long p = c->perm[q];
+            st[q] = p >= 0 ? E(getter(p)) : (dead < 1e300 ? E(dead) : E(getter(c->donor[q])));
+        }
         hipError_t e = hipMemcpy(dst, st.data(), sizeof(E) * n, hipMemcpyHostToDevice);
         return e == hipSuccess ? EP_OK : fail(EP_E_HIP, "hipMemcpy particle upload failed");
     };
     int r;
-#define PUT(dst, expr) do { if ((r = put(dst, [&](long p) { return (expr); })) != EP_OK) return r; } while (0)
+    const double COPY = 1e301;
+#define PUTD(dst, expr, dead) do { if ((r = put(dst, [&](long p) { return (expr); }, dead)) != EP_OK) return r; } while (0)
+#define PUT(dst, expr) PUTD(dst, expr, COPY)
     for (int i = 0; i < D; i++) { PUT(P.x[0][i], v->x[i + D * p]); PUT(P.x[1][i], v->x[i + D * p]); }
-    for (int i = 0; i < D; i++) PUT(P.v[i], v->v[i + D * p]);
-    for (int i = 0; i < DD; i++) PUT(P.F[i], v->F[i + DD * p]);
+    for (int i = 0; i < D; i++) PUTD(P.v[i], v->v[i + D * p], 0.0);
+    for (int i = 0; i < DD; i++) PUTD(P.F[i], v->F[i + DD * p], (i % (D + 1) == 0) ? 1.0 : 0.0);
     if (M.finite) {
         // symmetric part of bᵢⱼ: 2D xx,yy,xy; 3D xx,yy,zz,yz,xz,xy
         static const int i2[3][2] = {{0, 0}, {1, 1}, {0, 1}};
@@ -385,24 +393,26 @@ static int upload_fields(ep_ctx *c, const ep_particles_view *v, PState<T> &P) {
             const int a = D == 2 ? i2[k][0] : i3[k][0], bb = D == 2 ? i2[k][1] : i3[k][1];
             // the FP32 path stores B = b - I
             const double off = (c->fp32 && a == bb) ? 1.0 : 0.0;
-            PUT(P.b[k], 0.5 * (v->b[a + D * bb + DD * p] + v->b[bb + D * a + DD * p]) - off);
+            PUTD(P.b[k], 0.5 * (v->b[a + D * bb + DD * p] + v->b[bb + D * a + DD * p]) - off,
+                 (a == bb ? 1.0 : 0.0) - off);
         }
-        for (int k = 0; k < NS; k++) PUT(P.s[k], v->tau[k + NS * p]);
+        for (int k = 0; k < NS; k++) PUTD(P.s[k], v->tau[k + NS * p], 0.0);
     } else {
         for (int k = 0; k < (D == 2 ? 3 : 6); k++) PUT(P.b[k], (k < D && !c->fp32) ? 1.0 : 0.0);
-        for (int k = 0; k < NS; k++) PUT(P.s[k], v->sig[k + NS * p]);
+        for (int k = 0; k < NS; k++) PUTD(P.s[k], v->sig[k + NS * p], 0.0);
     }
-    PUT(P.epII, v->epII[0 + 2 * p]);
-    PUT(P.dlam, v->dlam ? v->dlam[p] : 0.0);
-    PUT(P.J, v->J[p]);
+    PUTD(P.epII, v->epII[0 + 2 * p], 0.0);
+    PUTD(P.dlam, v->dlam ? v->dlam[p] : 0.0, 0.0);
+    PUTD(P.J, v->J[p], 1.0);
     PUT(P.rho, M.finite ? v->rho0[p] : v->rho[p]);
-    PUT(P.Om0, v->Om0[p]);
+    PUTD(P.Om0, v->Om0[p], 0.0);
     PUT(P.c0, v->c0[p]);
     PUT(P.cr, v->cr[p]);
     PUT(P.phi, v->phi[p]);
     if (c->cfg.basis == EP_GIMPM) for (int i = 0; i < D; i++) PUT(P.l0[i], v->l0[i + D * p]);
-    if (M.tpic) for (int i = 0; i < DD; i++) PUT(P.gv[i], v->gradv[i + DD * p]);
+    if (M.tpic) for (int i = 0; i < DD; i++) PUTD(P.gv[i], v->gradv[i + DD * p], 0.0);
 #undef PUT
+#undef PUTD
     return EP_OK;
 }
 
@@ -418,12 +428,28 @@ extern "C" int ep_upload_particles(ep_ctx *c, const ep_particles_view *v) {
     if (!M.finite && (!v->sig || !v->rho)) return fail(EP_E_ARG, "infinitesimal deformation needs sig and rho");
     if (c->cfg.basis == EP_GIMPM && !v->l0) return fail(EP_E_ARG, "GIMP needs l0");
     if (M.tpic && !v->gradv) return fail(EP_E_ARG, "TPIC needs gradv");
-    // cell-sorted order
-    std::vector<long> key(n);
+    for (long p = 0; p < n; p++)
+        if (!(v->Om0[p] > 0.0)) return fail(EP_E_ARG, "particle volumes Om0 must be positive");
+    // cell-sorted order; in 3D every cell's run starts on a multiple of 8
+    // slots (padding in between), so the particles of a cell fill aligned
+    // 8-lane groups of a wave
+    std::vector<long> key(n), order(n);
     for (long p = 0; p < n; p++) key[p] = brick_key(c, v->x, p);
-    c->perm.resize(n);
-    std::iota(c->perm.begin(), c->perm.end(), 0L);
-    std::stable_sort(c->perm.begin(), c->perm.end(), [&](long a, long b) { return key[a] < key[b]; });
+    std::iota(order.begin(), order.end(), 0L);
+    std::stable_sort(order.begin(), order.end(), [&](long a, long b) { return key[a] < key[b]; });
+    c->perm.clear();
+    c->donor.clear();
+    const long G = D == 3 ? 8 : 1;
+    for (long a = 0; a < n;) {
+        long b = a;
+        while (b < n && key[order[b]] == key[order[a]]) b++;
+        for (long k = a; k < b; k++) { c->perm.push_back(order[k]); c->donor.push_back(order[k]); }
+        while (long(c->perm.size()) % G) { c->perm.push_back(-1); c->donor.push_back(order[a]); }
+        a = b;
+    }
+    c->nmp = n;
+    c->nslot = long(c->perm.size());
+    if (c->nslot > 0x7fffffffL) return fail(EP_E_ARG, "too many particle slots");
     {
         const int r = c->fp32 ? upload_fields<float>(c, v, c->psf) : upload_fields<double>(c, v, c->ps);
         if (r != EP_OK) return r;
@@ -438,10 +464,11 @@ extern "C" int ep_upload_particles(ep_ctx *c, const ep_particles_view *v) {
             return a;
         };
         SurrQ &q = c->mat.q;
-        q.idx = static_cast<int *>(qa(sizeof(int) * n));
+        const long ns = c->nslot;
+        q.idx = static_cast<int *>(qa(sizeof(int) * ns));
         bool ok = q.idx != nullptr;
-        for (int k = 0; k < 3; k++) ok = ok && (q.e[k] = static_cast<float *>(qa(sizeof(float) * n))) != nullptr;
-        for (int k = 0; k < 9; k++) ok = ok && (q.V[k] = static_cast<float *>(qa(sizeof(float) * n))) != nullptr;
+        for (int k = 0; k < 3; k++) ok = ok && (q.e[k] = static_cast<float *>(qa(sizeof(float) * ns))) != nullptr;
+        for (int k = 0; k < 9; k++) ok = ok && (q.V[k] = static_cast<float *>(qa(sizeof(float) * ns))) != nullptr;
         if (!ok) return fail(EP_E_HIP, "hipMalloc of the learned-return queue failed");
     }
     c->n0_h.assign(n, 0.1);
@@ -461,7 +488,7 @@ extern "C" int ep_upload_particles(ep_ctx *c, const ep_particles_view *v) {
     h.err = 0;
     hipFree(c->vmax_part);
     c->vmax_part = nullptr;
-    h.nblk = int((n + c->tb - 1) / c->tb);
+    h.nblk = int((c->nslot + c->tb - 1) / c->tb);
     HIPCHK(hipMalloc(&c->vmax_part, sizeof(unsigned long long) * 3 * size_t(h.nblk)));
     HIPCHK(hipMemset(c->vmax_part, 0, sizeof(unsigned long long) * 3 * size_t(h.nblk)));
     h.vmax_part = c->vmax_part;
@@ -482,7 +509,7 @@ extern "C" int ep_set_time(ep_ctx *c, double t) {
 template <typename T>
 static int download_fields(ep_ctx *c, ep_particles_view *v, const PState<T> &P, int par) {
     const int D = c->dim, NS = c->nstr, DD = D * D;
-    const long n = c->nmp;
+    const long n = c->nslot;
     const Mat &M = c->mat;
     std::vector<double> st(n);
     std::vector<T> stt(n);
@@ -498,17 +525,18 @@ static int download_fields(ep_ctx *c, ep_particles_view *v, const PState<T> &P, 
         return e == hipSuccess ? EP_OK : fail(EP_E_HIP, "hipMemcpy particle download failed");
     };
     int r;
-#define GET(src, stmt) do { if ((r = get(src)) != EP_OK) return r; for (long q = 0; q < n; q++) { const long p = c->perm[q]; const double val = st[q]; stmt; } } while (0)
+#define GET(src, stmt) do { if ((r = get(src)) != EP_OK) return r; for (long q = 0; q < n; q++) { const long p = c->perm[q]; if (p < 0) continue; const double val = st[q]; stmt; } } while (0)
     if (v->x) for (int i = 0; i < D; i++) GET(P.x[par][i], v->x[i + D * p] = val);
     if (v->v) for (int i = 0; i < D; i++) GET(P.v[i], v->v[i + D * p] = val);
     if (v->F) for (int i = 0; i < DD; i++) GET(P.F[i], v->F[i + DD * p] = val);
-    std::vector<double> J(n), tau;
+    const long np_ = c->nmp;
+    std::vector<double> J(np_), tau;
     GET(P.J, J[p] = val);
-    if (v->J) for (long p = 0; p < n; p++) v->J[p] = J[p];
-    if (v->Om) for (long p = 0; p < n; p++) v->Om[p] = J[p] * c->Om0_h[p];
+    if (v->J) for (long p = 0; p < np_; p++) v->J[p] = J[p];
+    if (v->Om) for (long p = 0; p < np_; p++) v->Om[p] = J[p] * c->Om0_h[p];
     if (M.finite) {
-        if (v->rho) for (long p = 0; p < n; p++) v->rho[p] = c->rho0_h[p] / J[p];
-        if (v->n) for (long p = 0; p < n; p++) v->n[p] = 1.0 - 1.0 / J[p] * (1.0 - c->n0_h[p]);
+        if (v->rho) for (long p = 0; p < np_; p++) v->rho[p] = c->rho0_h[p] / J[p];
+        if (v->n) for (long p = 0; p < np_; p++) v->n[p] = 1.0 - 1.0 / J[p] * (1.0 - c->n0_h[p]);
     }
     if (M.finite && v->b) {
         static const int i2[3][2] = {{0, 0}, {1, 1}, {0, 1}};
@@ -519,14 +547,14 @@ static int download_fields(ep_ctx *c, ep_particles_view *v, const PState<T> &P, 
             GET(P.b[k], (v->b[a + D * bb + DD * p] = val + off, v->b[bb + D * a + DD * p] = val + off));
         }
     }
-    tau.assign(size_t(NS) * n, 0.0);
+    tau.assign(size_t(NS) * np_, 0.0);
     for (int k = 0; k < NS; k++) GET(P.s[k], tau[k + NS * p] = val);
     if (M.finite) {
-        if (v->tau) std::memcpy(v->tau, tau.data(), sizeof(double) * NS * n);
-        if (v->sig) for (long p = 0; p < n; p++) for (int k = 0; k < NS; k++) v->sig[k + NS * p] = tau[k + NS * p] / J[p];
+        if (v->tau) std::memcpy(v->tau, tau.data(), sizeof(double) * NS * np_);
+        if (v->sig) for (long p = 0; p < np_; p++) for (int k = 0; k < NS; k++) v->sig[k + NS * p] = tau[k + NS * p] / J[p];
         if (v->eps) {
             // ϵᵢⱼ = tensor(Del \ τ, ½) (DP.jl:64)
-            for (long p = 0; p < n; p++) {
+            for (long p = 0; p < np_; p++) {
                 double y[6] = {0, 0, 0, 0, 0, 0}, yy[6];
                 for (int k = 0; k < NS; k++) y[k] = tau[k + NS * p];
                 // forward/back substitution with the host LU of Del
@@ -541,7 +569,7 @@ static int download_fields(ep_ctx *c, ep_particles_view *v, const PState<T> &P, 
             }
         }
     } else {
-        if (v->sig) std::memcpy(v->sig, tau.data(), sizeof(double) * NS * n);
+        if (v->sig) std::memcpy(v->sig, tau.data(), sizeof(double) * NS * np_);
     }
     if (v->epII) GET(P.epII, (v->epII[0 + 2 * p] = val, v->epII[1 + 2 * p] = val));
     if (v->dlam) GET(P.dlam, v->dlam[p] = val);
@@ -551,9 +579,9 @@ static int download_fields(ep_ctx *c, ep_particles_view *v, const PState<T> &P, 
     if (v->c0) GET(P.c0, v->c0[p] = val);
     if (v->cr) GET(P.cr, v->cr[p] = val);
     if (v->phi) GET(P.phi, v->phi[p] = val);
-    if (v->rho0) for (long p = 0; p < n; p++) v->rho0[p] = c->rho0_h[p];
-    if (v->Om0) for (long p = 0; p < n; p++) v->Om0[p] = c->Om0_h[p];
-    if (v->n0) for (long p = 0; p < n; p++) v->n0[p] = c->n0_h[p];
+    if (v->rho0) for (long p = 0; p < np_; p++) v->rho0[p] = c->rho0_h[p];
+    if (v->Om0) for (long p = 0; p < np_; p++) v->Om0[p] = c->Om0_h[p];
+    if (v->n0) for (long p = 0; p < np_; p++) v->n0[p] = c->n0_h[p];
     if (!M.finite && v->rho) GET(P.rho, v->rho[p] = val);
 #undef GET
     return EP_OK;
@@ -629,7 +657,7 @@ static mlp::SurrNet surr_net(ep_ctx *c);
 template <int D, int B, typename T, bool SURR>
 static void launch_step(ep_ctx *c, const PState<T> &P) {
     const int TB = c->tb;
-    const long n = c->nmp, nno = c->gs.nno;
+    const long n = c->nslot, nno = c->gs.nno;
     const dim3 gp((n + TB - 1) / TB), gg((nno + TB - 1) / TB);
     hipStream_t s = c->stream;
     Ctrl *ct = c->ctrl;
@@ -1075,7 +1103,7 @@ int ctx_set_block(ep_ctx *c, int tb) {
     HIPCHK(hipMemcpy(&h, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
     hipFree(c->vmax_part);
     c->vmax_part = nullptr;
-    h.nblk = int((c->nmp + tb - 1) / tb);
+    h.nblk = int((c->nslot + tb - 1) / tb);
     HIPCHK(hipMalloc(&c->vmax_part, sizeof(unsigned long long) * 3 * size_t(h.nblk)));
     HIPCHK(hipMemset(c->vmax_part, 0, sizeof(unsigned long long) * 3 * size_t(h.nblk)));
     h.vmax_part = c->vmax_part;
@@ -1096,7 +1124,7 @@ int ens_tables(ep_ensemble *e) {
         if ((r = ctx_set_block(c, tb)) != EP_OK) return r;
         if constexpr (std::is_same<T, float>::value) I[i] = Inst<T>{c->ctrl, c->geom, c->psf, c->gs, c->mat};
         else I[i] = Inst<T>{c->ctrl, c->geom, c->ps, c->gs, c->mat};
-        const long np = (c->nmp + tb - 1) / tb, ng = (c->gs.nno + tb - 1) / tb;
+        const long np = (c->nslot + tb - 1) / tb, ng = (c->gs.nno + tb - 1) / tb;
         pf[i] = int(pb);
         gf[i] = int(gb);
         for (long k = 0; k < np; k++) pi.push_back(i);
