@@ -35,8 +35,8 @@ int fail(int code, const std::string &msg) {
             return fail(EP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));        \
     } while (0)
 
-enum KernelId { K_BEGIN, K_ZERO, K_P2G, K_EULER, K_G2P, K_AUGV, K_DEFORM, K_FBAR, K_SURR, K_END, K_COUNT };
-const char *kKernelNames[K_COUNT] = {"begin", "zero_grid", "p2g", "euler", "g2p", "augv", "deform", "fbar",
+enum KernelId { K_BEGIN, K_ZERO, K_P2G, K_EULER, K_G2P, K_BIJ, K_AUGV, K_DEFORM, K_FBAR, K_SURR, K_END, K_COUNT };
+const char *kKernelNames[K_COUNT] = {"begin", "zero_grid", "p2g", "euler", "g2p", "bij", "augv", "deform", "fbar",
                                      "surrogate", "end"};
 
 // LU with partial pivoting of Del (same algorithm the device solve expects).
@@ -124,7 +124,7 @@ extern "C" int ep_create(const ep_config *cfg, ep_ctx **out) {
     if (cfg->dim != 2 && cfg->dim != 3) return fail(EP_E_CONFIG, "dim must be 2 or 3");
     if (cfg->basis < EP_BSMPM || cfg->basis > EP_SMPM)
         return fail(EP_E_CONFIG, "unsupported shape function basis");
-    if (cfg->transfer != EP_STD && cfg->transfer != EP_TPIC)
+    if (cfg->transfer != EP_STD && cfg->transfer != EP_TPIC && cfg->transfer != EP_APIC)
         return fail(EP_E_CONFIG, "unsupported transfer scheme");
     if (cfg->deform != EP_FINITE && cfg->deform != EP_INFINITESIMAL)
         return fail(EP_E_CONFIG, "unsupported deformation framework");
@@ -159,6 +159,7 @@ extern "C" int ep_create(const ep_config *cfg, ep_ctx **out) {
     factor_del(c->nstr, cfg->Del, M.lu);
     M.finite = cfg->deform == EP_FINITE;
     M.tpic = cfg->transfer == EP_TPIC;
+    M.apic = cfg->transfer == EP_APIC;
     M.musl = cfg->musl != 0;
     M.locking = cfg->locking != 0;
     M.plast = cfg->plast;
@@ -363,6 +364,7 @@ static int upload_fields(ep_ctx *c, const ep_particles_view *v, PState<T> &P) {
     P.Om0 = take(); P.c0 = take(); P.cr = take(); P.phi = take();
     if (c->cfg.basis == EP_GIMPM) for (int i = 0; i < D; i++) P.l0[i] = take();
     if (M.tpic) for (int i = 0; i < DD; i++) P.gv[i] = take();
+    if (M.apic) for (int i = 0; i < DD; i++) P.Bm[i] = take();
     if (oom) return fail(EP_E_HIP, "hipMalloc of particle fields failed");
     P.n = n;
     // host staging in device order.  A padding slot copies a particle of its
@@ -411,6 +413,7 @@ static int upload_fields(ep_ctx *c, const ep_particles_view *v, PState<T> &P) {
     PUT(P.phi, v->phi[p]);
     if (c->cfg.basis == EP_GIMPM) for (int i = 0; i < D; i++) PUT(P.l0[i], v->l0[i + D * p]);
     if (M.tpic) for (int i = 0; i < DD; i++) PUTD(P.gv[i], v->gradv[i + DD * p], 0.0);
+    if (M.apic) for (int i = 0; i < DD; i++) PUTD(P.Bm[i], v->Bij[i + DD * p], 0.0);
 #undef PUT
 #undef PUTD
     return EP_OK;
@@ -428,6 +431,7 @@ extern "C" int ep_upload_particles(ep_ctx *c, const ep_particles_view *v) {
     if (!M.finite && (!v->sig || !v->rho)) return fail(EP_E_ARG, "infinitesimal deformation needs sig and rho");
     if (c->cfg.basis == EP_GIMPM && !v->l0) return fail(EP_E_ARG, "GIMP needs l0");
     if (M.tpic && !v->gradv) return fail(EP_E_ARG, "TPIC needs gradv");
+    if (M.apic && !v->Bij) return fail(EP_E_ARG, "APIC needs Bij");
     for (long p = 0; p < n; p++)
         if (!(v->Om0[p] > 0.0)) return fail(EP_E_ARG, "particle volumes Om0 must be positive");
     // cell-sorted order; in 3D every cell's run starts on a multiple of 8
@@ -574,6 +578,7 @@ static int download_fields(ep_ctx *c, ep_particles_view *v, const PState<T> &P, 
     if (v->epII) GET(P.epII, (v->epII[0 + 2 * p] = val, v->epII[1 + 2 * p] = val));
     if (v->dlam) GET(P.dlam, v->dlam[p] = val);
     if (v->gradv && M.tpic) for (int i = 0; i < DD; i++) GET(P.gv[i], v->gradv[i + DD * p] = val);
+    if (v->Bij && M.apic) for (int i = 0; i < DD; i++) GET(P.Bm[i], v->Bij[i + DD * p] = val);
     if (v->l && c->cfg.basis == EP_GIMPM) for (int i = 0; i < D; i++) GET(P.l0[i], v->l[i + D * p] = val);
     if (v->l0 && c->cfg.basis == EP_GIMPM) for (int i = 0; i < D; i++) GET(P.l0[i], v->l0[i + D * p] = val);
     if (v->c0) GET(P.c0, v->c0[p] = val);
@@ -669,6 +674,7 @@ static void launch_step(ep_ctx *c, const PState<T> &P) {
     timed(c, K_P2G, [&] { EP_LAUNCH((k_p2g<D, B, T>), gp, TB, s, ct, g, P, G, M); });
     timed(c, K_EULER, [&] { EP_LAUNCH(k_euler<D>, gg, TB, s, ct, G, M); });
     timed(c, K_G2P, [&] { EP_LAUNCH((k_g2p<D, B, T>), gp, TB, s, ct, g, P, G, M); });
+    if (M.apic) timed(c, K_BIJ, [&] { EP_LAUNCH((k_bij<D, B, T>), gp, TB, s, ct, g, P, G); });
     if (M.musl) timed(c, K_AUGV, [&] { EP_LAUNCH(k_augv<D>, gg, TB, s, ct, G); });
     if (M.locking) timed(c, K_DEFORM, [&] { EP_LAUNCH((k_deform<D, B, T, true>), gp, TB, s, ct, g, P, G, M); });
     else timed(c, K_DEFORM, [&] { EP_LAUNCH((k_deform<D, B, T, false, SURR>), gp, TB, s, ct, g, P, G, M); });
@@ -1167,6 +1173,7 @@ int ens_capture(ep_ensemble *e) {
     EP_LAUNCH((kb_p2g<D, B, T>), e->npblk, tb, s, I, pm);
     EP_LAUNCH((kb_euler<D, T>), e->ngblk, tb, s, I, gm);
     EP_LAUNCH((kb_g2p<D, B, T>), e->npblk, tb, s, I, pm);
+    if (M.apic) EP_LAUNCH((kb_bij<D, B, T>), e->npblk, tb, s, I, pm);
     if (M.musl) EP_LAUNCH((kb_augv<D, T>), e->ngblk, tb, s, I, gm);
     if (M.locking) {
         EP_LAUNCH((kb_deform<D, B, T, true>), e->npblk, tb, s, I, pm);
@@ -1248,7 +1255,7 @@ int ens_run(ep_ensemble *e, int32_t nsteps, const ep_step_params *par, ep_step_r
 }
 
 struct HostState {
-    std::vector<double> f[24];
+    std::vector<double> f[25];
     ep_particles_view v{};
     double t = 0.0;
 };
@@ -1259,10 +1266,10 @@ int ens_save(ep_ctx *c, HostState &h) {
     const int D = c->dim, DD = D * D, NS = c->nstr;
     double **slot[] = {&h.v.x, &h.v.v, &h.v.l0, &h.v.l, &h.v.F, &h.v.b, &h.v.gradv, &h.v.eps, &h.v.sig, &h.v.tau,
                        &h.v.epII, &h.v.rho0, &h.v.rho, &h.v.Om0, &h.v.Om, &h.v.n0, &h.v.n, &h.v.J, &h.v.dJ,
-                       &h.v.c0, &h.v.cr, &h.v.phi, &h.v.dlam, &h.v.vmax};
+                       &h.v.c0, &h.v.cr, &h.v.phi, &h.v.dlam, &h.v.vmax, &h.v.Bij};
     const long len[] = {D * n, D * n, D * n, D * n, DD * n, DD * n, DD * n, DD * n, NS * n, NS * n, 2 * n,
-                        n, n, n, n, n, n, n, n, n, n, n, n, 3};
-    for (int k = 0; k < 24; k++) {
+                        n, n, n, n, n, n, n, n, n, n, n, n, 3, DD * n};
+    for (int k = 0; k < 25; k++) {
         h.f[k].assign(size_t(len[k]), 0.0);
         *slot[k] = h.f[k].data();
     }

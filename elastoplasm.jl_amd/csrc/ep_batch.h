@@ -69,6 +69,13 @@ __global__ void __launch_bounds__(256) kb_g2p(const Inst<T> *I, BlockMap m) {
     b_g2p<D, B, T>(in.c, in.g, in.P, in.G, in.M, long(blockIdx.x) - m.first[i]);
 }
 
+template <int D, int B, typename T>
+__global__ void __launch_bounds__(256) kb_bij(const Inst<T> *I, BlockMap m) {
+    const int i = m.inst[blockIdx.x];
+    const Inst<T> &in = I[i];
+    b_bij<D, B, T>(in.c, in.g, in.P, in.G, long(blockIdx.x) - m.first[i]);
+}
+
 template <int D, int B, typename T, bool LOCK>
 __global__ void __launch_bounds__(256) kb_deform(const Inst<T> *I, BlockMap m) {
     const int i = m.inst[blockIdx.x];

@@ -42,6 +42,7 @@ struct PState {
     T *Om0, *c0, *cr, *phi;
     T *l0[3];                  // GIMP half lengths
     T *gv[9];                  // ∇v (TPIC state)
+    T *Bm[9];                  // APIC Bᵢⱼ [i + D*j] (Bij.jl:14-26)
     long n;
 };
 
@@ -85,7 +86,7 @@ struct Mat {
     double E, nu;
     double Del[36];
     DelLU lu;
-    int finite, tpic, musl, locking, plast;
+    int finite, tpic, musl, locking, plast, apic;
     SurrQ q;
 };
 
@@ -637,7 +638,52 @@ __device__ __forceinline__ void b_zero_grid(const Ctrl *c, const GState &G, long
 template <int D>
 __global__ void k_zero_grid(const Ctrl *c, GState G) { b_zero_grid<D>(c, G, blockIdx.x); }
 
-// transform + std/tpic_{2,3}d_p2n (p2n.jl:12-21, 58-102, 140-191)
+// APIC's affine matrix B·D⁻¹ (p2n.jl:243-249, 267-273).  Dij_nd
+// (misc.jl:15-27) assigns Dᵢⱼ = N δδᵀ at every existing stencil node instead
+// of summing, so D is that of the last existing node in e2n order (y, x, z
+// nesting, e2n.jl): the largest existing index on each axis.  D⁻¹ = inv(D)
+// when |det D| > 1e-12, else the identity; a rank-one D gives the identity.
+template <int D, typename T>
+__device__ __forceinline__ void apic_affine(const Geom &g, const W<D, T> &w, const double *xp, const PState<T> &P,
+                                            long q, T (&A)[D][D]) {
+    double Bm[D][D], Dm[D][D], dl[D], N = 1.0;
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+        for (int j = 0; j < D; j++) Bm[i][j] = double(P.Bm[i + D * j][q]);
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+        const unsigned bits = (w.ok >> (4 * i)) & 0xFu;
+        const int k = bits ? 31 - __builtin_clz(bits) : 0;
+        const double wk = k == 0 ? double(w.w[i][0]) : (k == 1 ? double(w.w[i][1]) : (k == 2 ? double(w.w[i][2]) : double(w.w[i][3])));
+        N = i == 0 ? wk : N * wk;
+        dl[i] = g.X[i][clampi(w.base[i] + k, g.nno[i])] - xp[i];
+    }
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+        for (int j = 0; j < D; j++) Dm[i][j] = N * (dl[i] * dl[j]);
+    if (fabs(det<D, double>(Dm)) > 1.0e-12) {
+        double Di[D][D];
+        inv_lu<D>(Dm, Di);
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; j < D; j++) {
+                double s = Bm[i][0] * Di[0][j];
+#pragma unroll
+                for (int k = 1; k < D; k++) s += Bm[i][k] * Di[k][j];
+                A[i][j] = T(s);
+            }
+    } else {
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; j < D; j++) A[i][j] = T(Bm[i][j]);
+    }
+}
+
+// transform + std/tpic/apic_{2,3}d_p2n (p2n.jl:12-21, 58-102, 140-191, 231-280)
 template <int D, int B, typename T>
 __device__ __forceinline__ void b_p2g(const Ctrl *c, const Geom &g, const PState<T> &P, const GState &G, const Mat &M, long blk) {
     constexpr int NS = D == 2 ? 3 : 6;
@@ -679,8 +725,10 @@ __device__ __forceinline__ void b_p2g(const Ctrl *c, const Geom &g, const PState
 #pragma unroll
             for (int j = 0; j < D; j++) gv[i][j] = P.gv[i + D * j][q];
     }
+    // APIC scatters TPIC's form with B·D⁻¹ in place of ∇v
+    if (M.apic) apic_affine<D, T>(g, w, xp, P, q, gv);
     const T gz = T(c->g[D - 1]);
-    const bool tpic = M.tpic;
+    const bool tpic = M.tpic || M.apic;
     auto scatter = [&](const auto &sink) {
     scatter_walk<D, T>(g, w, sink, [&](long no, int li, T N, const T *dN, const int *ix) {
         sink(0, no, li, double(N * ms));
@@ -958,6 +1006,64 @@ __device__ __forceinline__ void b_g2p(Ctrl *c, const Geom &g, const PState<T> &P
 // kernel of one context: block b of the launch is block b of the particles / nodes
 template <int D, int B, typename T>
 __global__ void __launch_bounds__(256) k_g2p(Ctrl *c, Geom g, PState<T> P, GState G, Mat M) { b_g2p<D, B, T>(c, g, P, G, M, blockIdx.x); }
+
+// Bij (Bij.jl:14-26, n2p.jl:59-62): APIC's Bᵢⱼ = Σ N v_n Δnpᵀ from the
+// Euler grid velocity at the position before the PIC/FLIP update (the other
+// position buffer), between G2P and the mUSL velocity; without mUSL
+// n2p.jl:72-74 computes the same values a second time.
+template <int D, int B, typename T>
+__device__ __forceinline__ void b_bij(const Ctrl *c, const Geom &g, const PState<T> &P, const GState &G, long blk) {
+    __shared__ double smem[ACC_LDS];
+    __shared__ int sint[8];
+    if (!c->active) return;
+    const long p = blk * long(blockDim.x) + threadIdx.x;
+    const long q = p < P.n ? p : 0;
+    const bool live = p < P.n && P.Om0[q] != T(0);
+    const int par = c->parity;   // G2P wrote x into par ^ 1
+    double xp[D];
+    T lp[D];
+#pragma unroll
+    for (int i = 0; i < D; i++) { xp[i] = P.x[par][i][q]; lp[i] = B == 1 ? P.l0[i][q] : T(0); }
+    W<D, T> w;
+    weights<D, B, T>(g, xp, lp, w);
+    BlockBox<D> box;
+    box.begin((EP_LDS double *)smem, sint, live, w.base);
+    if (!live) w.retire(box.lo);
+    const bool staged = box.fits(D);
+    const double *src[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) src[d] = G.v[d];
+    if (staged) box.template stage<D>(g, src);
+    T Bacc[D][D];
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+        for (int j = 0; j < D; j++) Bacc[i][j] = T(0);
+    auto gather = [&](const auto &s) {
+        gather_walk<D, T>(g, w, s, [&](long no, int li, T N, const T *, const int *ix) {
+            T dl[D];
+#pragma unroll
+            for (int j = 0; j < D; j++) dl[j] = T(g.X[j][clampi(ix[j], g.nno[j])] - xp[j]);
+#pragma unroll
+            for (int i = 0; i < D; i++) {
+                const T vi = T(s(i, no, li));
+#pragma unroll
+                for (int j = 0; j < D; j++) Bacc[i][j] += N * (vi * dl[j]);
+            }
+        });
+    };
+    if (staged) gather(LdsSrc<D, D>{&box});
+    else gather(GlobSrc{src});
+    if (live) {
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; j < D; j++) P.Bm[i + D * j][p] = Bacc[i][j];
+    }
+}
+
+template <int D, int B, typename T>
+__global__ void __launch_bounds__(256) k_bij(const Ctrl *c, Geom g, PState<T> P, GState G) { b_bij<D, B, T>(c, g, P, G, blockIdx.x); }
 
 // augm_velocity (augm.jl:37-53) after mesh.v .= 0 (n2p.jl:67)
 template <int D>

@@ -191,6 +191,68 @@ __device__ __forceinline__ T det(const T (&A)[D][D]) {
     return (swaps & 1) ? -P : P;
 }
 
+// inv (p2n.jl:243-249): LU with partial pivoting as det above, then the
+// columns of the identity by forward and back substitution.
+template <int D>
+__device__ __forceinline__ void inv_lu(const double (&A)[D][D], double (&Ai)[D][D]) {
+    double M[D][D];
+    int piv[D];
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+        for (int j = 0; j < D; j++) M[i][j] = A[i][j];
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        int pk = k;
+        double mx = fabs(M[k][k]);
+#pragma unroll
+        for (int i = k + 1; i < D; i++)
+            if (fabs(M[i][k]) > mx) { mx = fabs(M[i][k]); pk = i; }
+        piv[k] = pk;
+#pragma unroll
+        for (int j = 0; j < D; j++)
+#pragma unroll
+            for (int i = k + 1; i < D; i++)
+                if (i == pk) { const double t = M[k][j]; M[k][j] = M[i][j]; M[i][j] = t; }
+        const double akk = M[k][k];
+        if (akk == 0.0) continue;
+        const double r = 1.0 / akk;
+#pragma unroll
+        for (int i = k + 1; i < D; i++) M[i][k] *= r;
+#pragma unroll
+        for (int j = k + 1; j < D; j++)
+#pragma unroll
+            for (int i = k + 1; i < D; i++) M[i][j] -= M[i][k] * M[k][j];
+    }
+#pragma unroll
+    for (int c = 0; c < D; c++) {
+        double y[D], x[D];
+#pragma unroll
+        for (int i = 0; i < D; i++) y[i] = i == c ? 1.0 : 0.0;
+#pragma unroll
+        for (int k = 0; k < D; k++)
+#pragma unroll
+            for (int i = k + 1; i < D; i++)
+                if (i == piv[k]) { const double t = y[k]; y[k] = y[i]; y[i] = t; }
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+            double s = y[i];
+#pragma unroll
+            for (int k = 0; k < i; k++) s -= M[i][k] * y[k];
+            y[i] = s;
+        }
+#pragma unroll
+        for (int i = D - 1; i >= 0; i--) {
+            double s = y[i];
+#pragma unroll
+            for (int k = i + 1; k < D; k++) s -= M[i][k] * x[k];
+            x[i] = s / M[i][i];
+        }
+#pragma unroll
+        for (int i = 0; i < D; i++) Ai[i][c] = x[i];
+    }
+}
+
 // Pre-factored Del for `Del \ τ` (DP.jl:64): LU with partial pivoting,
 // computed once on the host by the same algorithm.
 struct DelLU {

@@ -12,7 +12,7 @@ import copy
 import warnings
 
 BASIS = {"bsmpm": 0, "gimpm": 1, "smpm": 2}
-TRANSFER = {"std": 0, "tpic": 1}
+TRANSFER = {"std": 0, "tpic": 1, "apic": 2}
 DEFORM = {"finite": 0, "infinitesimal": 1}
 PLAST = {"DP": 1, "J2": 2}
 PRECISION = {"fp64": 0, "fp32": 1, "fp32_surrogate": 2}
@@ -70,8 +70,6 @@ def init_ops(dim: int, instr: dict) -> dict:
     if basis not in BASIS:
         raise ValueError(f"{basis} is not a supported shape function basis")
     fw = instr["fwrk"]
-    if fw["trsfr"] == "apic":
-        raise ValueError("apic is not part of the device path (see DESIGN.md)")
     if fw["trsfr"] not in TRANSFER:
         raise ValueError(f"{fw['trsfr']} is an unsupported transfer scheme")
     if fw["deform"] not in DEFORM:

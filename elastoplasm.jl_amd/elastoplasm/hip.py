@@ -40,7 +40,7 @@ class EpGridView(C.Structure):
 
 
 _PFIELDS = ["x", "v", "l0", "l", "F", "b", "gradv", "eps", "sig", "tau", "epII", "rho0", "rho", "Om0", "Om",
-            "n0", "n", "J", "dJ", "c0", "cr", "phi", "dlam", "vmax"]
+            "n0", "n", "J", "dJ", "c0", "cr", "phi", "dlam", "vmax", "Bij"]
 
 
 class EpParticlesView(C.Structure):
@@ -184,7 +184,7 @@ def particles_view(mpts: Point) -> EpParticlesView:
     m = {"x": mpts.x, "v": s.v, "l0": mpts.l0, "l": mpts.l, "F": s.F, "b": s.b, "gradv": s.gradv, "eps": s.eps,
          "sig": s.sig, "tau": s.tau, "epII": s.epII, "rho0": s.rho0, "rho": s.rho, "Om0": mpts.Om0, "Om": mpts.Om,
          "n0": mpts.n0, "n": mpts.n, "J": mpts.J, "dJ": mpts.dJ, "c0": s.c0, "cr": s.cr, "phi": s.phi,
-         "dlam": s.dlam, "vmax": mpts.vmax}
+         "dlam": s.dlam, "vmax": mpts.vmax, "Bij": mpts.Bij}
     for k, a in m.items():
         setattr(v, k, _ptr(a))
     return v

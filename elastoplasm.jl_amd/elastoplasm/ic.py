@@ -268,7 +268,7 @@ def setup_mpts(mesh, cmpr, geom) -> Point:
     )
     return Point(ndim=d, nmp=nmp, vmax=np.zeros(d), x=np.asfortranarray(xp.copy()), l0=l0,
                  l=l0.copy(order="F"), n0=n0, n=n0.copy(), Om0=v0.copy(), Om=v0.copy(),
-                 dJ=np.ones(nmp), J=np.ones(nmp), s=s, z0=xp[-1, :].copy())
+                 dJ=np.ones(nmp), J=np.ones(nmp), s=s, z0=xp[-1, :].copy(), Bij=fzeros((d, d, nmp)))
 
 
 def setup_time(instr, te=0.0, tg=0.0, tep=0.0) -> NT:

@@ -83,9 +83,11 @@ class Point:
     J: np.ndarray
     s: Solid
     z0: np.ndarray = field(default=None)
+    Bij: np.ndarray | None = None      # Bᵢⱼ [dim, dim, nmp], APIC state (types.jl:81-83)
     # materialised stencil, CPU oracle only
     phi: np.ndarray | None = None      # ϕ∂ϕ [nn, nmp, dim+1]
     dnp: np.ndarray | None = None      # Δnp [nn, dim, nmp]
+    Dij: np.ndarray | None = None      # Dᵢⱼ [dim, dim, nmp] (APIC)
     p2e: np.ndarray | None = None
     p2n: np.ndarray | None = None
 
@@ -97,4 +99,8 @@ class Point:
             self.dnp = fzeros((nn, d, self.nmp))
             self.p2e = np.zeros(self.nmp, dtype=np.int64)
             self.p2n = np.zeros((nn, self.nmp), dtype=np.int64, order="F")
+        if self.Dij is None:
+            self.Dij = fzeros((self.ndim, self.ndim, self.nmp))
+        if self.Bij is None:
+            self.Bij = fzeros((self.ndim, self.ndim, self.nmp))
         return self

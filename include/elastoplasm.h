@@ -41,7 +41,7 @@ enum ep_status {
 };
 
 enum ep_basis { EP_BSMPM = 0, EP_GIMPM = 1, EP_SMPM = 2 };         /* instr.basis.which */
-enum ep_transfer { EP_STD = 0, EP_TPIC = 1 };                      /* instr.fwrk.trsfr */
+enum ep_transfer { EP_STD = 0, EP_TPIC = 1, EP_APIC = 2 };         /* instr.fwrk.trsfr */
 enum ep_deform { EP_FINITE = 0, EP_INFINITESIMAL = 1 };            /* instr.fwrk.deform */
 enum ep_plast { EP_PLAST_NONE = 0, EP_DP = 1, EP_J2 = 2 };         /* instr.plast.constitutive */
 enum ep_precision { EP_FP64 = 0, EP_FP32 = 1, EP_FP32_SURROGATE = 2 };
@@ -100,6 +100,8 @@ typedef struct ep_particles_view {
     double *phi;          /* [nmp] mpts.s.ϕ */
     double *dlam;         /* [nmp] mpts.s.Δλ */
     double *vmax;         /* [dim] mpts.vmax */
+    double *Bij;          /* [dim,dim,nmp] mpts.Bᵢⱼ (APIC state, types.jl:81-83; Bij.jl:14-26);
+                           * mpts.Dᵢⱼ and Δnp are recomputed each step (misc.jl:15-27) */
 } ep_particles_view;
 
 /* Nodal fields for parity checks (download only), [dim,nno] / [nno]. */

@@ -191,6 +191,17 @@ void or_lu_solve(int n, const double *A, const double *b, double *x) {
     }
 }
 
+/* inv (Bij via p2n.jl:249, 273): columns of the identity solved through the
+ * same LU with partial pivoting */
+void or_inv(int d, const double *A, double *Ai) {
+    for (int j = 0; j < d; j++) {
+        double e[3] = {0.0, 0.0, 0.0}, x[3];
+        e[j] = 1.0;
+        or_lu_solve(d, A, e, x);
+        for (int i = 0; i < d; i++) Ai[i + d * j] = x[i];
+    }
+}
+
 /* y = Del * x, Del [n,n] column-major */
 static void matvec(int n, const double *A, const double *x, double *y) {
     for (int i = 0; i < n; i++) {
@@ -388,12 +399,35 @@ static void delta_np(or_point *mp, const or_mesh *me) {
         }
 }
 
+/* Dij_nd (misc.jl:15-27): Δnp as Δnp_nd, and Dᵢⱼ *assigned* (not summed)
+ * at every existing node, so the last existing node of the stencil wins. */
+static void dij_nd(or_point *mp, const or_mesh *me) {
+    const int dim = me->dim, nn = me->nn, dd = dim * dim;
+    const int64_t nmp = mp->nmp;
+    PAR_FOR
+    for (int64_t p = 0; p < nmp; p++)
+        for (int k = 0; k < nn; k++) {
+            int64_t no = mp->p2n[k + nn * p];
+            if (no == 0) continue;
+            no -= 1;
+            double dl[3];
+            for (int i = 0; i < dim; i++) {
+                dl[i] = me->x[i + dim * no] - mp->x[i + dim * p];
+                mp->dnp[k + nn * (i + dim * p)] = dl[i];
+            }
+            const double N = PHI(k, p, 0);
+            for (int j = 0; j < dim; j++)
+                for (int i = 0; i < dim; i++) mp->Dij[i + dim * j + dd * p] = N * (dl[i] * dl[j]);
+        }
+}
+
 /* shpfun (shpfun.jl:74-88) */
 int or_shpfun(or_point *mp, const or_mesh *me, const or_instr *in) {
     p2e2n(mp, me);
     memset(mp->phi, 0, sizeof(double) * (size_t)me->nn * mp->nmp * (me->dim + 1));
     int r = basis(mp, me, in);
     if (in->trsfr == OR_TPIC) delta_np(mp, me);
+    else if (in->trsfr == OR_APIC) dij_nd(mp, me);
     return r;
 }
 
@@ -401,8 +435,30 @@ int or_shpfun(or_point *mp, const or_mesh *me, const or_instr *in) {
 /* mapsto                                                                    */
 /* ------------------------------------------------------------------------- */
 
-/* transform + std/tpic_{2,3}d_p2n + p2n wrapper (p2n.jl:12-21, 58-102,
- * 140-191, 284-296) */
+/* APIC's B·D⁻¹·Δnp for one node (p2n.jl:243-249, 267-273): D⁻¹ = inv(D)
+ * when |det D| > 1e-12, else the identity; Julia evaluates B*D⁻¹*x as
+ * B*(D⁻¹*x). */
+static void apic_dinv(int dim, const double *D, double *Di, int *ident) {
+    *ident = !(fabs(or_det(dim, D)) > 1.0e-12);
+    if (!*ident) or_inv(dim, D, Di);
+}
+static void apic_bw(int dim, const double *B, const double *Di, int ident, const double *x, double *o) {
+    double w[3];
+    for (int i = 0; i < dim; i++) {
+        if (ident) { w[i] = x[i]; continue; }
+        double s = Di[i] * x[0];
+        for (int j = 1; j < dim; j++) s += Di[i + dim * j] * x[j];
+        w[i] = s;
+    }
+    for (int i = 0; i < dim; i++) {
+        double s = B[i] * w[0];
+        for (int j = 1; j < dim; j++) s += B[i + dim * j] * w[j];
+        o[i] = s;
+    }
+}
+
+/* transform + std/tpic/apic_{2,3}d_p2n + p2n wrapper (p2n.jl:12-21, 58-102,
+ * 140-191, 231-280, 284-296) */
 void or_p2n(or_point *mp, or_mesh *me, const double *g, const or_instr *in) {
     const int dim = me->dim, nn = me->nn, nstr = dim == 2 ? 3 : 6;
     const int64_t nmp = mp->nmp, nno = me->nno[dim];
@@ -416,12 +472,15 @@ void or_p2n(or_point *mp, or_mesh *me, const double *g, const or_instr *in) {
     memset(me->m, 0, sizeof(double) * nno);
     memset(me->p, 0, sizeof(double) * dim * nno);
     memset(me->oobf, 0, sizeof(double) * dim * nno);
-    const int tpic = in->trsfr == OR_TPIC;
+    const int tpic = in->trsfr == OR_TPIC, apic = in->trsfr == OR_APIC;
     PAR_FOR
     for (int64_t p = 0; p < nmp; p++) {
         double ms = mp->rho[p] * mp->Om[p], Om = mp->Om[p];
         const double *s = mp->sig + nstr * p;
         const double *gv = mp->gradv + dim * dim * p;
+        double Di[9];
+        int ident = 1;
+        if (apic) apic_dinv(dim, mp->Dij + dim * dim * p, Di, &ident);
         if (dim == 2) {
             double vx = mp->v[0 + 2 * p], vy = mp->v[1 + 2 * p];
             double px = ms * vx, py = ms * vy;
@@ -436,6 +495,11 @@ void or_p2n(or_point *mp, or_mesh *me, const double *g, const or_instr *in) {
                     double dx = mp->dnp[k + nn * (0 + 2 * p)], dy = mp->dnp[k + nn * (1 + 2 * p)];
                     atom_add(&me->p[0 + 2 * no], N * ms * (vx + gv[0] * dx + gv[2] * dy));
                     atom_add(&me->p[1 + 2 * no], N * ms * (vy + gv[1] * dx + gv[3] * dy));
+                } else if (apic) {
+                    double x[2] = {mp->dnp[k + nn * (0 + 2 * p)], mp->dnp[k + nn * (1 + 2 * p)]}, bw[2];
+                    apic_bw(2, mp->Bij + 4 * p, Di, ident, x, bw);
+                    atom_add(&me->p[0 + 2 * no], N * ms * (vx + bw[0]));
+                    atom_add(&me->p[1 + 2 * no], N * ms * (vy + bw[1]));
                 } else {
                     atom_add(&me->p[0 + 2 * no], N * px);
                     atom_add(&me->p[1 + 2 * no], N * py);
@@ -460,6 +524,13 @@ void or_p2n(or_point *mp, or_mesh *me, const double *g, const or_instr *in) {
                     atom_add(&me->p[0 + 3 * no], N * ms * (vx + gv[0] * dx + gv[3] * dy + gv[6] * dz));
                     atom_add(&me->p[1 + 3 * no], N * ms * (vy + gv[1] * dx + gv[4] * dy + gv[7] * dz));
                     atom_add(&me->p[2 + 3 * no], N * ms * (vz + gv[2] * dx + gv[5] * dy + gv[8] * dz));
+                } else if (apic) {
+                    double x[3] = {mp->dnp[k + nn * (0 + 3 * p)], mp->dnp[k + nn * (1 + 3 * p)],
+                                   mp->dnp[k + nn * (2 + 3 * p)]}, bw[3];
+                    apic_bw(3, mp->Bij + 9 * p, Di, ident, x, bw);
+                    atom_add(&me->p[0 + 3 * no], N * ms * (vx + bw[0]));
+                    atom_add(&me->p[1 + 3 * no], N * ms * (vy + bw[1]));
+                    atom_add(&me->p[2 + 3 * no], N * ms * (vz + bw[2]));
                 } else {
                     atom_add(&me->p[0 + 3 * no], N * px);
                     atom_add(&me->p[1 + 3 * no], N * py);
@@ -538,6 +609,26 @@ void or_n2p(or_point *mp, or_mesh *me, double dt, const or_instr *in) {
     mp->vmax[0] = vm0;
     mp->vmax[1] = vm1;
     if (dim == 3) mp->vmax[2] = vm2;
+    /* Bij (Bij.jl:14-26, n2p.jl:59-62): Bᵢⱼ = Σ ϕ (v_n Δnpᵀ) from the grid
+     * velocity of the Euler solve; without mUSL n2p.jl:72-74 computes the
+     * same values a second time. */
+    if (in->trsfr == OR_APIC) {
+        const int dd = dim * dim;
+        PAR_FOR
+        for (int64_t p = 0; p < nmp; p++) {
+            double B[9] = {0.0};
+            for (int k = 0; k < nn; k++) {
+                int64_t no = mp->p2n[k + nn * p];
+                if (no == 0) continue;
+                no -= 1;
+                const double N = PHI(k, p, 0);
+                for (int j = 0; j < dim; j++)
+                    for (int i = 0; i < dim; i++)
+                        B[i + dim * j] += N * (me->v[i + dim * no] * mp->dnp[k + nn * (j + dim * p)]);
+            }
+            memcpy(mp->Bij + dd * p, B, sizeof(double) * dd);
+        }
+    }
     if (in->musl) {
         memset(me->p, 0, sizeof(double) * dim * nno);
         memset(me->v, 0, sizeof(double) * dim * nno);

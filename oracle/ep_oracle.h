@@ -30,7 +30,7 @@ extern "C" {
 #endif
 
 enum { OR_BSMPM = 0, OR_GIMPM = 1, OR_SMPM = 2 };
-enum { OR_STD = 0, OR_TPIC = 1 };
+enum { OR_STD = 0, OR_TPIC = 1, OR_APIC = 2 };
 enum { OR_FINITE = 0, OR_INFINITESIMAL = 1 };
 enum { OR_PLAST_NONE = 0, OR_DP = 1, OR_J2 = 2 };
 enum { OR_OK = 0, OR_E_DOMAIN = -3 };
@@ -66,6 +66,7 @@ typedef struct or_point {
     double *epII;            /* ϵpII [2,nmp] */
     double *sig, *tau;       /* σᵢ, τᵢ [nstr,nmp] */
     double *gradv, *dF, *F, *b, *eps, *omg; /* [dim,dim,nmp] */
+    double *Bij, *Dij;       /* APIC Bᵢⱼ, Dᵢⱼ [dim,dim,nmp] (types.jl:81-83) */
 } or_point;
 
 /* instr.fwrk / instr.basis / cmpr (src/home/api/require.jl:34-88,
@@ -83,6 +84,8 @@ int  or_version(void);
 void or_set_threads(int n);
 
 /* get_dt (src/home/program/utils/physics.jl:21-26) */
+double or_det(int d, const double *A);
+void or_inv(int d, const double *A, double *Ai);
 double or_get_dt(or_point *mp, const or_mesh *me, double c, double t, double T);
 
 /* shpfun (src/home/program/workflow/shpfun/shpfun.jl:74-88) */

@@ -34,7 +34,8 @@ class OrPoint(C.Structure):
                 ("n0", _dp), ("n", _dp), ("Om0", _dp), ("Om", _dp), ("dJ", _dp), ("J", _dp),
                 ("v", _dp), ("rho0", _dp), ("rho", _dp), ("c0", _dp), ("cr", _dp), ("phif", _dp), ("dlam", _dp),
                 ("epII", _dp), ("sig", _dp), ("tau", _dp),
-                ("gradv", _dp), ("dF", _dp), ("F", _dp), ("b", _dp), ("eps", _dp), ("omg", _dp)]
+                ("gradv", _dp), ("dF", _dp), ("F", _dp), ("b", _dp), ("eps", _dp), ("omg", _dp),
+                ("Bij", _dp), ("Dij", _dp)]
 
 
 class OrInstr(C.Structure):
@@ -58,6 +59,9 @@ def load():
             build()
         L = C.CDLL(str(LIB))
         L.or_get_dt.restype = C.c_double
+        L.or_det.restype = C.c_double
+        L.or_det.argtypes = [C.c_int, _dp]
+        L.or_inv.argtypes = [C.c_int, _dp, _dp]
         L.or_get_dt.argtypes = [C.POINTER(OrPoint), C.POINTER(OrMesh), C.c_double, C.c_double, C.c_double]
         L.or_step.argtypes = [C.POINTER(OrPoint), C.POINTER(OrMesh), C.POINTER(OrInstr), _dp, C.c_double, C.c_int]
         L.or_shpfun.argtypes = [C.POINTER(OrPoint), C.POINTER(OrMesh), C.POINTER(OrInstr)]
@@ -135,6 +139,7 @@ class Oracle:
         pt.epII, pt.sig, pt.tau = _p(s.epII), _p(s.sig), _p(s.tau)
         pt.gradv, pt.dF, pt.F, pt.b, pt.eps, pt.omg = (_p(s.gradv), _p(s.dF), _p(s.F), _p(s.b), _p(s.eps),
                                                        _p(s.omg))
+        pt.Bij, pt.Dij = _p(mpts.Bij), _p(mpts.Dij)
         self.pt = pt
         ins = OrInstr()
         ins.basis, ins.trsfr, ins.musl, ins.deform = ops["basis"], ops["transfer"], ops["musl"], ops["deform"]

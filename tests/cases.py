@@ -39,7 +39,7 @@ FIELDS = ["x", "v", "F", "b", "tau", "sig", "epII", "J", "dlam"]
 def field(mpts, name):
     s = mpts.s
     return {"x": mpts.x, "v": s.v, "F": s.F, "b": s.b, "tau": s.tau, "sig": s.sig, "epII": s.epII[0],
-            "J": mpts.J, "dlam": s.dlam, "gradv": s.gradv}[name]
+            "J": mpts.J, "dlam": s.dlam, "gradv": s.gradv, "Bij": mpts.Bij}[name]
 
 
 def relerr(a, b) -> float:

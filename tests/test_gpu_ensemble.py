@@ -18,8 +18,8 @@ pytestmark = pytest.mark.gpu
 EMU = "cpucheck" in os.environ.get("EP_HIP_LIB", "")
 
 
-def _build(n, seed, precision, scales=(1, 2)):
-    lst, draw = ep.ic_ensemble(n, seed, scales=scales, power=1.0, precision=precision)
+def _build(n, seed, precision, scales=(1, 2), **kw):
+    lst, draw = ep.ic_ensemble(n, seed, scales=scales, power=1.0, precision=precision, **kw)
     return lst, draw
 
 
@@ -31,8 +31,9 @@ def _devices(lst):
     return out
 
 
-def test_ensemble_matches_single_contexts_and_oracle():
-    lst, _ = _build(3 if EMU else 8, 1, "fp64")
+@pytest.mark.parametrize("trsfr,musl", [("tpic", False), ("apic", True)])
+def test_ensemble_matches_single_contexts_and_oracle(trsfr, musl):
+    lst, _ = _build(3 if EMU else 8, 1, "fp64", trsfr=trsfr, musl=musl)
     nsteps = (2, 1) if EMU else (6, 4)
     devs = _devices(lst)
     ens = hip.Ensemble([d for _, d in devs])
@@ -55,6 +56,8 @@ def test_ensemble_matches_single_contexts_and_oracle():
         e3 = relerr(st.mpts.x - x0, ref.mpts.x - x0)
         print(f"instance {i} nmp {ic.mpts.nmp}: vs alone {e1:.1e}, vs oracle stress {e2:.1e} displacement {e3:.1e}")
         assert e1 <= 1e-12 and e2 <= 1e-9 and e3 <= 1e-9
+        if trsfr == "apic":
+            assert relerr(st.mpts.Bij, ref.mpts.Bij) <= 1e-9
     ens.close()
     for _, d in devs:
         d.close()

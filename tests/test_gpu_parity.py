@@ -68,6 +68,12 @@ CASES_2D = [
     dict(deform="infinitesimal", trsfr="std", musl=True, locking=True),
     dict(deform="infinitesimal", trsfr="std", musl=True, locking=False),
     dict(deform="infinitesimal", trsfr="tpic", musl=False, locking=False),
+    # the APIC rows of the reference's slump test matrix (test_slump.jl:8-15)
+    dict(deform="finite", trsfr="apic", musl=True, locking=True),
+    dict(deform="finite", trsfr="apic", musl=True, locking=False),
+    dict(deform="infinitesimal", trsfr="apic", musl=True, locking=True),
+    dict(deform="infinitesimal", trsfr="apic", musl=True, locking=False),
+    dict(deform="finite", trsfr="apic", musl=False, locking=True),
 ]
 
 
@@ -77,6 +83,8 @@ def test_one_step_2d_bsmpm(case):
     fields = ["x", "v", "F", "J", "epII", "dlam"] + (["b", "tau"] if case["deform"] == "finite" else ["sig"])
     if case["trsfr"] == "tpic":
         fields.append("gradv")
+    if case["trsfr"] == "apic":
+        fields.append("Bij")
     _one_step(ic, cfg, 40, fields)
 
 
@@ -91,9 +99,10 @@ def test_one_step_2d_j2():
     _one_step(ic, cfg, 40, ["x", "v", "F", "J", "b", "tau", "epII"])
 
 
-def test_one_step_3d():
-    ic, cfg = slump(3)
-    _one_step(ic, cfg, 10, ["x", "v", "F", "J", "b", "tau", "epII", "dlam"])
+@pytest.mark.parametrize("trsfr", ["std", "apic"])
+def test_one_step_3d(trsfr):
+    ic, cfg = slump(3, trsfr=trsfr)
+    _one_step(ic, cfg, 10, ["x", "v", "F", "J", "b", "tau", "epII", "dlam"] + (["Bij"] if trsfr == "apic" else []))
 
 
 def test_one_step_3d_gimp_infinitesimal_j2():
@@ -102,10 +111,11 @@ def test_one_step_3d_gimp_infinitesimal_j2():
     _one_step(ic, cfg, 10, ["x", "v", "F", "J", "sig", "epII"])
 
 
-def test_c1_100_steps():
-    """C1: 2D slump, DP c=20 kPa, phi=30, rho 2650, bsmpm, mUSL-FLIP, finite,
-    CFL 0.5, 100 steps from t=0."""
-    ic, cfg = slump(2, C_pf=1.0, cmpr=C1)
+@pytest.mark.parametrize("trsfr", ["std", "apic"])
+def test_c1_100_steps(trsfr):
+    """C1: 2D slump, DP c=20 kPa, phi=30, rho 2650, bsmpm, mUSL-FLIP (or
+    APIC), finite, CFL 0.5, 100 steps from t=0."""
+    ic, cfg = slump(2, C_pf=1.0, cmpr=C1, trsfr=trsfr)
     x0 = ic.mpts.x.copy(order="F")
     ref, dev = clone(ic), clone(ic)
     o = Oracle(ref.mesh, ref.mpts, ref.cmpr, cfg.instr)
@@ -122,8 +132,9 @@ def test_c1_100_steps():
     assert du <= RUN_TOL and dtau <= RUN_TOL
 
 
-def test_3d_20_steps():
-    ic, cfg = slump(3, C_pf=1.0, cmpr=C1)
+@pytest.mark.parametrize("trsfr", ["std", "apic"])
+def test_3d_20_steps(trsfr):
+    ic, cfg = slump(3, C_pf=1.0, cmpr=C1, trsfr=trsfr)
     x0 = ic.mpts.x.copy(order="F")
     ref, dev = clone(ic), clone(ic)
     o = Oracle(ref.mesh, ref.mpts, ref.cmpr, cfg.instr)
