@@ -126,6 +126,14 @@ extern "C" int ep_create(const ep_config *cfg, ep_ctx **out) {
         return fail(EP_E_CONFIG, "unsupported shape function basis");
     if (cfg->transfer != EP_STD && cfg->transfer != EP_TPIC && cfg->transfer != EP_APIC)
         return fail(EP_E_CONFIG, "unsupported transfer scheme");
+    // update.jl:7-29: the domain kernels of each framework; GIMP only
+    {
+        const int dm = cfg->domain;
+        const bool ok = dm == EP_UNDEFORMED ||
+                        (cfg->basis == EP_GIMPM && cfg->deform == EP_FINITE && dm >= EP_DETF && dm <= EP_UII) ||
+                        (cfg->basis == EP_GIMPM && cfg->deform == EP_INFINITESIMAL && dm >= EP_DETDF && dm <= EP_DUII);
+        if (!ok) return fail(EP_E_CONFIG, "domain update does not match the basis / deformation framework");
+    }
     if (cfg->deform != EP_FINITE && cfg->deform != EP_INFINITESIMAL)
         return fail(EP_E_CONFIG, "unsupported deformation framework");
     if (cfg->plast < EP_PLAST_NONE || cfg->plast > EP_J2) return fail(EP_E_CONFIG, "InvalidReturnMapping");
@@ -160,6 +168,7 @@ extern "C" int ep_create(const ep_config *cfg, ep_ctx **out) {
     M.finite = cfg->deform == EP_FINITE;
     M.tpic = cfg->transfer == EP_TPIC;
     M.apic = cfg->transfer == EP_APIC;
+    M.domain = cfg->basis == EP_GIMPM ? cfg->domain : EP_UNDEFORMED;
     M.musl = cfg->musl != 0;
     M.locking = cfg->locking != 0;
     M.plast = cfg->plast;
@@ -362,7 +371,8 @@ static int upload_fields(ep_ctx *c, const ep_particles_view *v, PState<T> &P) {
     for (int i = 0; i < NS; i++) P.s[i] = take();
     P.epII = take(); P.dlam = take(); P.J = take(); P.rho = take();
     P.Om0 = take(); P.c0 = take(); P.cr = take(); P.phi = take();
-    if (c->cfg.basis == EP_GIMPM) for (int i = 0; i < D; i++) P.l0[i] = take();
+    if (c->cfg.basis == EP_GIMPM) for (int i = 0; i < D; i++) P.l[i] = take();
+    if (M.domain != EP_UNDEFORMED) for (int i = 0; i < D; i++) P.l0[i] = take();
     if (M.tpic) for (int i = 0; i < DD; i++) P.gv[i] = take();
     if (M.apic) for (int i = 0; i < DD; i++) P.Bm[i] = take();
     if (oom) return fail(EP_E_HIP, "hipMalloc of particle fields failed");
@@ -411,7 +421,10 @@ static int upload_fields(ep_ctx *c, const ep_particles_view *v, PState<T> &P) {
     PUT(P.c0, v->c0[p]);
     PUT(P.cr, v->cr[p]);
     PUT(P.phi, v->phi[p]);
-    if (c->cfg.basis == EP_GIMPM) for (int i = 0; i < D; i++) PUT(P.l0[i], v->l0[i + D * p]);
+    // `undeformed`: ℓ = ℓ₀ at every step, so ℓ comes from ℓ₀
+    if (c->cfg.basis == EP_GIMPM)
+        for (int i = 0; i < D; i++) PUT(P.l[i], M.domain != EP_UNDEFORMED ? v->l[i + D * p] : v->l0[i + D * p]);
+    if (M.domain != EP_UNDEFORMED) for (int i = 0; i < D; i++) PUT(P.l0[i], v->l0[i + D * p]);
     if (M.tpic) for (int i = 0; i < DD; i++) PUTD(P.gv[i], v->gradv[i + DD * p], 0.0);
     if (M.apic) for (int i = 0; i < DD; i++) PUTD(P.Bm[i], v->Bij[i + DD * p], 0.0);
 #undef PUT
@@ -430,6 +443,7 @@ extern "C" int ep_upload_particles(ep_ctx *c, const ep_particles_view *v) {
     if (M.finite && (!v->b || !v->tau)) return fail(EP_E_ARG, "finite deformation needs b and tau");
     if (!M.finite && (!v->sig || !v->rho)) return fail(EP_E_ARG, "infinitesimal deformation needs sig and rho");
     if (c->cfg.basis == EP_GIMPM && !v->l0) return fail(EP_E_ARG, "GIMP needs l0");
+    if (M.domain != EP_UNDEFORMED && !v->l) return fail(EP_E_ARG, "a GIMP domain update needs l");
     if (M.tpic && !v->gradv) return fail(EP_E_ARG, "TPIC needs gradv");
     if (M.apic && !v->Bij) return fail(EP_E_ARG, "APIC needs Bij");
     for (long p = 0; p < n; p++)
@@ -579,8 +593,9 @@ static int download_fields(ep_ctx *c, ep_particles_view *v, const PState<T> &P, 
     if (v->dlam) GET(P.dlam, v->dlam[p] = val);
     if (v->gradv && M.tpic) for (int i = 0; i < DD; i++) GET(P.gv[i], v->gradv[i + DD * p] = val);
     if (v->Bij && M.apic) for (int i = 0; i < DD; i++) GET(P.Bm[i], v->Bij[i + DD * p] = val);
-    if (v->l && c->cfg.basis == EP_GIMPM) for (int i = 0; i < D; i++) GET(P.l0[i], v->l[i + D * p] = val);
-    if (v->l0 && c->cfg.basis == EP_GIMPM) for (int i = 0; i < D; i++) GET(P.l0[i], v->l0[i + D * p] = val);
+    if (v->l && c->cfg.basis == EP_GIMPM) for (int i = 0; i < D; i++) GET(P.l[i], v->l[i + D * p] = val);
+    if (v->l0 && c->cfg.basis == EP_GIMPM)
+        for (int i = 0; i < D; i++) GET(M.domain != EP_UNDEFORMED ? P.l0[i] : P.l[i], v->l0[i + D * p] = val);
     if (v->c0) GET(P.c0, v->c0[p] = val);
     if (v->cr) GET(P.cr, v->cr[p] = val);
     if (v->phi) GET(P.phi, v->phi[p] = val);

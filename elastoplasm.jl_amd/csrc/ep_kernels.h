@@ -40,7 +40,8 @@ struct PState {
     T *epII, *dlam, *J;
     T *rho;                    // ρ₀ (finite: ρ = ρ₀/J) or the constant ρ (infinitesimal)
     T *Om0, *c0, *cr, *phi;
-    T *l0[3];                  // GIMP half lengths
+    T *l[3];                   // GIMP half lengths ℓ the basis uses
+    T *l0[3];                  // ℓ₀, kept for domain updates other than `undeformed`
     T *gv[9];                  // ∇v (TPIC state)
     T *Bm[9];                  // APIC Bᵢⱼ [i + D*j] (Bij.jl:14-26)
     long n;
@@ -86,7 +87,7 @@ struct Mat {
     double E, nu;
     double Del[36];
     DelLU lu;
-    int finite, tpic, musl, locking, plast, apic;
+    int finite, tpic, musl, locking, plast, apic, domain;
     SurrQ q;
 };
 
@@ -698,7 +699,7 @@ __device__ __forceinline__ void b_p2g(const Ctrl *c, const Geom &g, const PState
     double xp[D];
     T lp[D];
 #pragma unroll
-    for (int i = 0; i < D; i++) { xp[i] = P.x[par][i][q]; lp[i] = B == 1 ? P.l0[i][q] : T(0); }
+    for (int i = 0; i < D; i++) { xp[i] = P.x[par][i][q]; lp[i] = B == 1 ? P.l[i][q] : T(0); }
     W<D, T> w;
     if (!weights<D, B, T>(g, xp, lp, w) && live) atomicOr(const_cast<unsigned *>(&c->err), ERR_DOMAIN);
     BlockBox<D> box;
@@ -937,7 +938,7 @@ __device__ __forceinline__ void b_g2p(Ctrl *c, const Geom &g, const PState<T> &P
     double xp[D];
     T lp[D];
 #pragma unroll
-    for (int i = 0; i < D; i++) { xp[i] = P.x[par][i][q]; lp[i] = B == 1 ? P.l0[i][q] : T(0); }
+    for (int i = 0; i < D; i++) { xp[i] = P.x[par][i][q]; lp[i] = B == 1 ? P.l[i][q] : T(0); }
     W<D, T> w;
     weights<D, B, T>(g, xp, lp, w);
     BlockBox<D> box;
@@ -1023,7 +1024,7 @@ __device__ __forceinline__ void b_bij(const Ctrl *c, const Geom &g, const PState
     double xp[D];
     T lp[D];
 #pragma unroll
-    for (int i = 0; i < D; i++) { xp[i] = P.x[par][i][q]; lp[i] = B == 1 ? P.l0[i][q] : T(0); }
+    for (int i = 0; i < D; i++) { xp[i] = P.x[par][i][q]; lp[i] = B == 1 ? P.l[i][q] : T(0); }
     W<D, T> w;
     weights<D, B, T>(g, xp, lp, w);
     BlockBox<D> box;
@@ -1411,6 +1412,56 @@ __device__ __forceinline__ void deform_f(const PState<T> &P, long q, T dt, const
         }
 }
 
+// GIMP domain update (domain.jl:1-52) chosen by basis.how per framework
+// (update.jl:7-29), after the deformation update and before F-bar
+// (update.jl:43-49).  Restated from intent: domain.jl reads mpts.Fᵢⱼ and
+// mpts.ΔFᵢⱼ, which are fields of mpts.s.  dF is ΔF (FP32 path: ΔF - I), Fn
+// the updated F and J = det Fn; U = n diag(√λ) nᵀ from eigen(AᵀA).
+enum { DOM_UNDEFORMED = 0, DOM_DETF = 1, DOM_FII = 2, DOM_UII = 3, DOM_DETDF = 4, DOM_DFII = 5, DOM_DUII = 6 };
+template <int D, typename T>
+__device__ __forceinline__ void domain_update(const PState<T> &P, const Mat &M, long p, const T (&dF)[D][D],
+                                              const T (&Fn)[D][D], T J) {
+    const int how = M.domain;
+    if (how == DOM_UNDEFORMED) return;       // ℓ stays ℓ₀
+    const bool inc = how >= DOM_DETDF;       // infinitesimal: scales the current ℓ
+    T A[D][D];
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+        for (int j = 0; j < D; j++)
+            A[i][j] = inc ? ((kDev<T> && i == j) ? T(1) + dF[i][j] : dF[i][j]) : Fn[i][j];
+    T f[D];
+    if (how == DOM_DETF || how == DOM_DETDF) {
+        const T d = how == DOM_DETF ? J : (kDev<T> ? T(1) + det1m<D, T>(dF) : det<D, T>(dF));
+#pragma unroll
+        for (int i = 0; i < D; i++) f[i] = d;
+    } else if (how == DOM_FII || how == DOM_DFII) {
+#pragma unroll
+        for (int i = 0; i < D; i++) f[i] = A[i][i];
+    } else {
+        T Cm[D][D], V[D][D];
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; j < D; j++) {
+                T s = A[0][i] * A[0][j];
+#pragma unroll
+                for (int k = 1; k < D; k++) s += A[k][i] * A[k][j];
+                Cm[i][j] = s;
+            }
+        sym_eig<D, T>(Cm, V);
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+            T u = T(0);
+#pragma unroll
+            for (int k = 0; k < D; k++) u += V[i][k] * sqrt(Cm[k][k]) * V[i][k];
+            f[i] = u;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < D; i++) P.l[i][p] = f[i] * (inc ? P.l[i][p] : P.l0[i][p]);
+}
+
 // finite/infinitesimal_deform (deform.jl:1-45).
 // LOCK = false: the whole particle update (deform + elast + plast) in one
 //   pass.
@@ -1429,7 +1480,7 @@ __device__ __forceinline__ void b_deform(Ctrl *c, const Geom &g, const PState<T>
     double xp[D];
     T lp[D];
 #pragma unroll
-    for (int i = 0; i < D; i++) { xp[i] = P.x[par][i][q]; lp[i] = B == 1 ? P.l0[i][q] : T(0); }
+    for (int i = 0; i < D; i++) { xp[i] = P.x[par][i][q]; lp[i] = B == 1 ? P.l[i][q] : T(0); }
     W<D, T> w;
     weights<D, B, T>(g, xp, lp, w);
     BlockBox<D> box;
@@ -1473,6 +1524,7 @@ __device__ __forceinline__ void b_deform(Ctrl *c, const Geom &g, const PState<T>
                 if (M.tpic) P.gv[i + D * j][p] = gv[i][j];
             }
         P.J[p] = J;
+        if (B == 1) domain_update<D, T>(P, M, p, dF, Fn, J);
         constitutive<D, T, SURR>(c, P, M, p, dF);
     }
 }
@@ -1498,7 +1550,7 @@ __device__ __forceinline__ void b_fbar(Ctrl *c, const Geom &g, const PState<T> &
     double xp[D];
     T lp[D];
 #pragma unroll
-    for (int i = 0; i < D; i++) { xp[i] = P.x[par][i][q]; lp[i] = B == 1 ? P.l0[i][q] : T(0); }
+    for (int i = 0; i < D; i++) { xp[i] = P.x[par][i][q]; lp[i] = B == 1 ? P.l[i][q] : T(0); }
     W<D, T> w;
     weights<D, B, T>(g, xp, lp, w);
     BlockBox<D> box;
@@ -1535,6 +1587,7 @@ __device__ __forceinline__ void b_fbar(Ctrl *c, const Geom &g, const PState<T> &
             if (M.tpic) P.gv[i + D * j][p] = gv[i][j];
         }
     P.J[p] = J;
+    if (B == 1) domain_update<D, T>(P, M, p, dF, Fn, J);
     T acc = T(0);
     // FP64: Σ N ΔJ_n/ΔJ_p as volumetric.jl:27-32.  FP32: the nodal values are
     // δJ_n = ΔJ_n - 1 and Σ N δJ_n is accumulated, so that with Σ N = 1

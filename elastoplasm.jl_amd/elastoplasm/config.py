@@ -13,6 +13,9 @@ import warnings
 
 BASIS = {"bsmpm": 0, "gimpm": 1, "smpm": 2}
 TRANSFER = {"std": 0, "tpic": 1, "apic": 2}
+# GIMP domain updates (domain.jl:1-52) by framework (update.jl:7-29)
+DOMAIN_FINITE = {"undeformed": 0, "detFij": 1, "Fii": 2, "Uii": 3}
+DOMAIN_INFINITESIMAL = {"undeformed": 0, "detΔFij": 4, "ΔFii": 5, "ΔUii": 6}
 DEFORM = {"finite": 0, "infinitesimal": 1}
 PLAST = {"DP": 1, "J2": 2}
 PRECISION = {"fp64": 0, "fp32": 1, "fp32_surrogate": 2}
@@ -74,10 +77,15 @@ def init_ops(dim: int, instr: dict) -> dict:
         raise ValueError(f"{fw['trsfr']} is an unsupported transfer scheme")
     if fw["deform"] not in DEFORM:
         raise ValueError(f"{fw['deform']} is an unsupported deformation framework")
-    if basis == "gimpm" and instr["basis"].get("how") != "undeformed":
-        # update.jl:7-28 leaves domain! undefined for how=nothing, and the
-        # other `how` kernels read fields Point does not have (domain.jl:7-52)
-        raise ValueError("gimpm needs basis.how = 'undeformed'")
+    domain = 0
+    if basis == "gimpm":
+        # update.jl:7-28: the `how` kernels of each framework; any other value
+        # leaves domain! undefined (an UndefVarError at setup)
+        how = instr["basis"].get("how")
+        table = DOMAIN_FINITE if fw["deform"] == "finite" else DOMAIN_INFINITESIMAL
+        if how not in table:
+            raise ValueError(f"gimpm: basis.how = {how!r} is not a {fw['deform']} domain update")
+        domain = table[how]
     cons = instr["plast"]["constitutive"]
     if cons not in PLAST:
         raise ValueError(f"InvalidReturnMapping: {cons}")
@@ -95,4 +103,5 @@ def init_ops(dim: int, instr: dict) -> dict:
         "C_pf": float(fw["C_pf"]),
         "damping": float(fw["damping"]),
         "precision": PRECISION[prec],
+        "domain": domain,
     }

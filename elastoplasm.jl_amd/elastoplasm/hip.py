@@ -32,7 +32,7 @@ class EpConfig(C.Structure):
                 ("h", C.c_double * 3), ("x0", C.c_double * 3), ("xB", (C.c_double * 2) * 3),
                 ("C_pf", C.c_double), ("damping", C.c_double), ("Kc", C.c_double), ("Gc", C.c_double),
                 ("Hp", C.c_double), ("c", C.c_double), ("Del", C.c_double * 36),
-                ("surrogate_tol", C.c_double), ("reserved", C.c_int32 * 8)]
+                ("surrogate_tol", C.c_double), ("domain", C.c_int32), ("reserved", C.c_int32 * 7)]
 
 
 class EpGridView(C.Structure):
@@ -175,6 +175,7 @@ def make_config(mesh: Mesh, mpts: Point, cmpr, instr) -> EpConfig:
     for k, v in enumerate(Del):
         cfg.Del[k] = v
     cfg.surrogate_tol = float(instr.get("surrogate_tol", 1e-3))
+    cfg.domain = ops.get("domain", 0)
     return cfg
 
 

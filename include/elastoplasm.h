@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define EP_ABI_VERSION 1
+#define EP_ABI_VERSION 2   /* 2: ep_particles_view.Bij, ep_config.domain */
 
 enum ep_status {
     EP_OK = 0,
@@ -45,6 +45,12 @@ enum ep_transfer { EP_STD = 0, EP_TPIC = 1, EP_APIC = 2 };         /* instr.fwrk
 enum ep_deform { EP_FINITE = 0, EP_INFINITESIMAL = 1 };            /* instr.fwrk.deform */
 enum ep_plast { EP_PLAST_NONE = 0, EP_DP = 1, EP_J2 = 2 };         /* instr.plast.constitutive */
 enum ep_precision { EP_FP64 = 0, EP_FP32 = 1, EP_FP32_SURROGATE = 2 };
+/* instr.basis.how, the GIMP domain update (domain.jl:1-52, update.jl:7-29):
+ * finite: undeformed, detFij, Fii, Uii; infinitesimal: undeformed, detΔFij,
+ * ΔFii, ΔUii.  The reference's kernels other than `undeformed` read
+ * mpts.Fᵢⱼ / mpts.ΔFᵢⱼ, which live on mpts.s (types.jl:38-63), so they do
+ * not run there; they are restated from their intent. */
+enum ep_domain { EP_UNDEFORMED = 0, EP_DETF = 1, EP_FII = 2, EP_UII = 3, EP_DETDF = 4, EP_DFII = 5, EP_DUII = 6 };
 
 /* Setup-time description of one problem: instr + mesh scalars + cmpr.
  * Replaces kwargser's cairn construction (kwargser.jl:49-59) and the
@@ -71,7 +77,8 @@ typedef struct ep_config {
     double c;             /* cmpr.c, elastic wave speed for get_dt */
     double Del[36];       /* cmpr.Del, nstr x nstr column-major */
     double surrogate_tol; /* EP_FP32_SURROGATE: yield-residual tolerance */
-    int32_t reserved[8];
+    int32_t domain;       /* GIMP domain update, an ep_domain value */
+    int32_t reserved[7];
 } ep_config;
 
 /* Pointers into mesh (types.jl:141-167). */

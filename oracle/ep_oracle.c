@@ -655,6 +655,41 @@ void or_n2p(or_point *mp, or_mesh *me, double dt, const or_instr *in) {
 /* elastoplast                                                               */
 /* ------------------------------------------------------------------------- */
 
+/* GIMP domain update (domain.jl:1-52), chosen by basis.how per framework
+ * (update.jl:7-29), after the deformation update and before F-bar
+ * (update.jl:43-49).  Restated from intent: domain.jl reads mpts.Fᵢⱼ and
+ * mpts.ΔFᵢⱼ, which are mpts.s fields (types.jl:38-63), here s.F and s.dF.
+ * U = n diag(√λ) nᵀ from eigen(FᵀF). */
+static void domain_point(or_point *mp, int dim, int64_t p, int how) {
+    const int dd = dim * dim;
+    double *l = mp->l + dim * p;
+    const double *l0 = mp->l0 + dim * p, *F = mp->F + dd * p, *dF = mp->dF + dd * p;
+    if (how == OR_DOM_UNDEFORMED) {
+        for (int i = 0; i < dim; i++) l[i] = l0[i];
+    } else if (how == OR_DOM_DETF || how == OR_DOM_DETDF) {
+        const double d = or_det(dim, how == OR_DOM_DETF ? F : dF);
+        for (int i = 0; i < dim; i++) l[i] = (how == OR_DOM_DETF ? l0[i] : l[i]) * d;
+    } else if (how == OR_DOM_FII || how == OR_DOM_DFII) {
+        const double *A = how == OR_DOM_FII ? F : dF;
+        for (int i = 0; i < dim; i++) l[i] = (how == OR_DOM_FII ? l0[i] : l[i]) * A[i + dim * i];
+    } else {
+        const double *A = how == OR_DOM_UII ? F : dF;
+        double C[9], lam[3], V[9];
+        for (int i = 0; i < dim; i++)
+            for (int j = 0; j < dim; j++) {
+                double s = A[0 + dim * i] * A[0 + dim * j];
+                for (int k = 1; k < dim; k++) s += A[k + dim * i] * A[k + dim * j];
+                C[i + dim * j] = s;
+            }
+        or_sym_eig(dim, C, lam, V);
+        for (int i = 0; i < dim; i++) {
+            double u = 0.0;
+            for (int k = 0; k < dim; k++) u += V[i + dim * k] * sqrt(lam[k]) * V[i + dim * k];
+            l[i] = u * (how == OR_DOM_UII ? l0[i] : l[i]);
+        }
+    }
+}
+
 /* finite/infinitesimal_deform (deform.jl:1-45), undeformed (domain.jl:1-6),
  * ΔJn/ΔJs/ΔJp (volumetric.jl:1-35), update (update.jl:43-64) */
 void or_update(or_point *mp, or_mesh *me, double dt, const or_instr *in) {
@@ -687,8 +722,7 @@ void or_update(or_point *mp, or_mesh *me, double dt, const or_instr *in) {
     }
     if (in->basis == OR_GIMPM) {
         PAR_FOR
-        for (int64_t p = 0; p < nmp; p++)
-            for (int i = 0; i < dim; i++) mp->l[i + dim * p] = mp->l0[i + dim * p];
+        for (int64_t p = 0; p < nmp; p++) domain_point(mp, dim, p, in->domain);
     }
     if (in->locking) {
         memset(me->dJ, 0, sizeof(double) * dim * nno);

@@ -31,6 +31,8 @@ extern "C" {
 
 enum { OR_BSMPM = 0, OR_GIMPM = 1, OR_SMPM = 2 };
 enum { OR_STD = 0, OR_TPIC = 1, OR_APIC = 2 };
+enum { OR_DOM_UNDEFORMED = 0, OR_DOM_DETF = 1, OR_DOM_FII = 2, OR_DOM_UII = 3, OR_DOM_DETDF = 4, OR_DOM_DFII = 5,
+       OR_DOM_DUII = 6 };
 enum { OR_FINITE = 0, OR_INFINITESIMAL = 1 };
 enum { OR_PLAST_NONE = 0, OR_DP = 1, OR_J2 = 2 };
 enum { OR_OK = 0, OR_E_DOMAIN = -3 };
@@ -74,7 +76,7 @@ typedef struct or_point {
 typedef struct or_instr {
     int32_t basis, trsfr, musl, deform, plast, locking;
     int32_t nthreads;        /* 1 = sequential reference order */
-    int32_t pad;
+    int32_t domain;          /* GIMP domain update, OR_DOM_* */
     double C_pf, damping;
     double Kc, Gc, Hp, c;
     double Del[36];          /* [nstr,nstr] column-major */

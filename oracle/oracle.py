@@ -40,7 +40,7 @@ class OrPoint(C.Structure):
 
 class OrInstr(C.Structure):
     _fields_ = [("basis", C.c_int32), ("trsfr", C.c_int32), ("musl", C.c_int32), ("deform", C.c_int32),
-                ("plast", C.c_int32), ("locking", C.c_int32), ("nthreads", C.c_int32), ("pad", C.c_int32),
+                ("plast", C.c_int32), ("locking", C.c_int32), ("nthreads", C.c_int32), ("domain", C.c_int32),
                 ("C_pf", C.c_double), ("damping", C.c_double), ("Kc", C.c_double), ("Gc", C.c_double),
                 ("Hp", C.c_double), ("c", C.c_double), ("Del", C.c_double * 36)]
 
@@ -144,6 +144,7 @@ class Oracle:
         ins = OrInstr()
         ins.basis, ins.trsfr, ins.musl, ins.deform = ops["basis"], ops["transfer"], ops["musl"], ops["deform"]
         ins.plast, ins.locking, ins.nthreads = ops["plast"], ops["locking"], threads
+        ins.domain = ops.get("domain", 0)
         ins.C_pf, ins.damping = ops["C_pf"], ops["damping"]
         ins.Kc, ins.Gc, ins.Hp, ins.c = cmpr.Kc, cmpr.Gc, cmpr.Hp, cmpr.c
         for k, v in enumerate(np.asarray(cmpr.Del).ravel(order="F")):

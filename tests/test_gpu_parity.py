@@ -94,6 +94,38 @@ def test_one_step_2d_other_bases(basis):
     _one_step(ic, cfg, 30, ["x", "v", "F", "J", "b", "tau", "epII"])
 
 
+DOMAINS = [("finite", "detFij"), ("finite", "Fii"), ("finite", "Uii"), ("infinitesimal", "detΔFij"),
+           ("infinitesimal", "ΔFii"), ("infinitesimal", "ΔUii")]
+
+
+@pytest.mark.parametrize("deform,how", DOMAINS)
+def test_one_step_2d_gimp_domain(deform, how):
+    """GIMP with each domain update of domain.jl (basis.how)."""
+    ic, cfg = slump(2, basis="gimpm", deform=deform, how=how)
+    stress = ["b", "tau"] if deform == "finite" else ["sig"]
+    _one_step(ic, cfg, 30, ["x", "v", "F", "J", "epII", "l"] + stress)
+
+
+@pytest.mark.parametrize("deform,how", [("finite", "Uii"), ("infinitesimal", "ΔUii")])
+def test_gimp_domain_3d_and_60_steps(deform, how):
+    """3D GIMP one step, and 60 2D steps from rest, with the stretch-tensor
+    domain updates."""
+    ic, cfg = slump(3, basis="gimpm", deform=deform, how=how)
+    stress = ["b", "tau"] if deform == "finite" else ["sig"]
+    _one_step(ic, cfg, 5, ["x", "v", "F", "J", "l"] + stress)
+    ic, cfg = slump(2, basis="gimpm", deform=deform, how=how)
+    x0 = ic.mpts.x.copy(order="F")
+    ref, dev = clone(ic), clone(ic)
+    Oracle(ref.mesh, ref.mpts, ref.cmpr, cfg.instr).run(60, 100.0, plast=True)
+    r, _ = _device_run(dev, cfg, 60, 100.0)
+    assert r.steps == 60
+    du = relerr(dev.mpts.x - x0, ref.mpts.x - x0)
+    dl = relerr(dev.mpts.l - ic.mpts.l, ref.mpts.l - ic.mpts.l)
+    print(f"GIMP {how} 60 steps: displacement {du:.2e}, change of l {dl:.2e}")
+    assert np.abs(ref.mpts.l - ic.mpts.l).max() > 0
+    assert du <= RUN_TOL and dl <= RUN_TOL
+
+
 def test_one_step_2d_j2():
     ic, cfg = slump(2, plast="J2", cmpr=dict(c0=8.0e3, cr=4.0e3, Hp=1.0e4))
     _one_step(ic, cfg, 40, ["x", "v", "F", "J", "b", "tau", "epII"])
