@@ -36,6 +36,7 @@ from elastoplasm import hip  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 SURR_TOL = 3e-5      # scaled residual a learned-return prediction must meet
+RESORT = 0           # steps between device re-sorts of the particles into cell order (0: never)
 MAT = dict(rho0=2650.0, E=1.0e6, nu=0.3, c0=20.0e3, cr=20.0e3, phi0=math.radians(30.0), phir=math.radians(30.0),
            Hp=0.0)
 
@@ -188,6 +189,8 @@ def main():
     ap.add_argument("--surr-steps", type=int, default=6000, help="Adam steps (batch 65536)")
     ap.add_argument("--ensemble-steps", type=int, default=1000, help="steps of the 256-slump ensemble (0: skip)")
     ap.add_argument("--surr-tol", type=float, default=SURR_TOL, help="residual tolerance of the learned return")
+    ap.add_argument("--resort", type=int, default=RESORT,
+                    help="re-sort the particles into cell order every K steps (0: never)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the step would shard by spatial "
@@ -226,6 +229,7 @@ def main():
         instr = copy.deepcopy(cfg.instr)
         instr["ops"]["precision"] = {"fp64": 0, "fp32": 1, "fp32_surrogate": 2}[precision]
         dev = hip.Device(st.mesh, st.mpts, st.cmpr, instr)
+        dev.set_resort(args.resort)
         if precision == "fp32_surrogate":
             dev.attach_surrogate(net, args.surr_tol)
         if args.warmup:
@@ -321,7 +325,7 @@ def main():
                    "transfer": "mUSL-FLIP", "deform": "finite", "locking": True, "plast": "DP",
                    "cohesion": (f"log-normal random field, mean {GRF['mean']:g} Pa, CoV {GRF['cov']}, "
                                 f"correlation length {GRF['ell']} m, seed {GRF['seed']}" if dim == 3 else "uniform"),
-                   "precision": "fp64", "parallelism": "single-gpu"},
+                   "precision": "fp64", "resort_every": args.resort, "parallelism": "single-gpu"},
         "roofline": roof,
         "step_roofline": step_roof,
         "kernels_us": {k: v * 1000 for k, v in per_kernel.items()},

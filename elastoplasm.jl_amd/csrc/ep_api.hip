@@ -7,6 +7,9 @@
 #include "../../include/elastoplasm.h"
 #include "ep_mlp.h"
 #include "ep_batch.h"
+#ifndef EP_CPU_CHECK
+#include <hipcub/hipcub.hpp>
+#endif
 
 #include <algorithm>
 #include <cmath>
@@ -35,9 +38,10 @@ int fail(int code, const std::string &msg) {
             return fail(EP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));        \
     } while (0)
 
-enum KernelId { K_BEGIN, K_ZERO, K_P2G, K_EULER, K_G2P, K_BIJ, K_AUGV, K_DEFORM, K_FBAR, K_SURR, K_END, K_COUNT };
+enum KernelId { K_BEGIN, K_ZERO, K_P2G, K_EULER, K_G2P, K_BIJ, K_AUGV, K_DEFORM, K_FBAR, K_SURR, K_END, K_SORT,
+                K_COUNT };
 const char *kKernelNames[K_COUNT] = {"begin", "zero_grid", "p2g", "euler", "g2p", "bij", "augv", "deform", "fbar",
-                                     "surrogate", "end"};
+                                     "surrogate", "end", "resort"};
 
 // LU with partial pivoting of Del (same algorithm the device solve expects).
 void factor_del(int n, const double *A, DelLU &f) {
@@ -91,6 +95,21 @@ struct ep_ctx {
     std::vector<long> donor;           // device slot -> a particle of the same cell (for padding)
     std::vector<double> n0_h, rho0_h, Om0_h;
     bool part_ready = false;
+    // periodic re-sort into brick-major cell order (k_sort_keys): slot list
+    // on the device, spare field buffers of each element type, sort scratch
+    int resort_every = 0;
+    long since_sort = 0;
+    bool in_ensemble = false;          // an ensemble holds this context's field pointers
+    int *perm_d = nullptr;             // device copy of perm, permuted with the fields
+    bool perm_dirty = false;
+    double *spare_d = nullptr;
+    float *spare_f = nullptr;
+    int *spare_i = nullptr;
+    std::vector<void *> sort_allocs;
+    unsigned *sort_key[2] = {nullptr, nullptr};
+    int *sort_val = nullptr, *sort_idx = nullptr;
+    void *sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
     // launch shape: threads per particle/node block and cells per sort brick
     int tb = EP_BLOCK;
     long brick[3] = {4, 2, 4};
@@ -205,6 +224,7 @@ extern "C" void ep_destroy(ep_ctx *c) {
     hipFree(c->bc_dev);
     for (void *a : c->part_allocs) hipFree(a);
     for (void *a : c->q_allocs) hipFree(a);
+    for (void *a : c->sort_allocs) hipFree(a);
     hipFree(c->ctrl);
     hipFree(c->dt_log);
     hipFree(c->vmax_part);
@@ -375,6 +395,18 @@ static int upload_fields(ep_ctx *c, const ep_particles_view *v, PState<T> &P) {
     if (M.domain != EP_UNDEFORMED) for (int i = 0; i < D; i++) P.l0[i] = take();
     if (M.tpic) for (int i = 0; i < DD; i++) P.gv[i] = take();
     if (M.apic) for (int i = 0; i < DD; i++) P.Bm[i] = take();
+    // re-sort: the device slot list and one spare buffer per element type
+    auto take_i = [&]() -> int * {
+        void *a = nullptr;
+        if (hipMalloc(&a, sizeof(int) * n) != hipSuccess) { oom = true; return nullptr; }
+        c->part_allocs.push_back(a);
+        return static_cast<int *>(a);
+    };
+    c->spare_d = take_d();
+    c->spare_f = nullptr;
+    if constexpr (std::is_same<T, float>::value) c->spare_f = take();
+    c->perm_d = take_i();
+    c->spare_i = take_i();
     if (oom) return fail(EP_E_HIP, "hipMalloc of particle fields failed");
     P.n = n;
     // host staging in device order.  A padding slot copies a particle of its
@@ -427,6 +459,20 @@ static int upload_fields(ep_ctx *c, const ep_particles_view *v, PState<T> &P) {
     if (M.domain != EP_UNDEFORMED) for (int i = 0; i < D; i++) PUT(P.l0[i], v->l0[i + D * p]);
     if (M.tpic) for (int i = 0; i < DD; i++) PUTD(P.gv[i], v->gradv[i + DD * p], 0.0);
     if (M.apic) for (int i = 0; i < DD; i++) PUTD(P.Bm[i], v->Bij[i + DD * p], 0.0);
+    {
+        std::vector<int> pi(n);
+        for (long q = 0; q < n; q++) pi[q] = int(c->perm[q]);
+        if (hipMemcpy(c->perm_d, pi.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess)
+            return fail(EP_E_HIP, "hipMemcpy particle upload failed");
+    }
+    for (void *a : c->sort_allocs) hipFree(a);
+    c->sort_allocs.clear();
+    c->sort_key[0] = c->sort_key[1] = nullptr;
+    c->sort_val = c->sort_idx = nullptr;
+    c->sort_tmp = nullptr;
+    c->sort_tmp_bytes = 0;
+    c->since_sort = 0;
+    c->perm_dirty = false;
 #undef PUT
 #undef PUTD
     return EP_OK;
@@ -512,6 +558,14 @@ extern "C" int ep_upload_particles(ep_ctx *c, const ep_particles_view *v) {
     h.vmax_part = c->vmax_part;
     HIPCHK(hipMemcpy(c->ctrl, &h, sizeof(Ctrl), hipMemcpyHostToDevice));
     c->part_ready = true;
+    return EP_OK;
+}
+
+extern "C" int ep_set_resort(ep_ctx *c, int32_t every) {
+    if (!c) return fail(EP_E_ARG, "null context");
+    if (every < 0) return fail(EP_E_ARG, "ep_set_resort: every must be >= 0");
+    c->resort_every = every;
+    c->since_sort = 0;
     return EP_OK;
 }
 
@@ -616,6 +670,12 @@ extern "C" int ep_download_particles(ep_ctx *c, ep_particles_view *v) {
     const Mat &M = c->mat;
     Ctrl h{};
     HIPCHK(hipMemcpy(&h, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+    if (c->perm_dirty) {
+        std::vector<int> pi(size_t(c->nslot));
+        HIPCHK(hipMemcpy(pi.data(), c->perm_d, sizeof(int) * size_t(c->nslot), hipMemcpyDeviceToHost));
+        for (long q = 0; q < c->nslot; q++) c->perm[q] = pi[q];
+        c->perm_dirty = false;
+    }
     {
         const int r = c->fp32 ? download_fields<float>(c, v, c->psf, h.parity)
                               : download_fields<double>(c, v, c->ps, h.parity);
@@ -717,6 +777,99 @@ static void dispatch_step_t(ep_ctx *c, const PState<T> &P) {
     }
 }
 
+// Re-sort (k_sort_keys, k_permute): a stable radix sort of the slots by
+// their brick-major cell key, then every particle field gathered through the
+// sorted slot list into a spare buffer that takes the field's place.  The
+// slot count stays; padding slots move to the end.
+template <int D, typename T>
+static int resort_t(ep_ctx *c, PState<T> &P) {
+    const long n = c->nslot;
+    hipStream_t s = c->stream;
+    const int TB = 256;
+    const dim3 gp((n + TB - 1) / TB);
+    if (!c->sort_idx) {
+        auto alloc = [&](size_t bytes) -> void * {
+            void *a = nullptr;
+            if (hipMalloc(&a, bytes) != hipSuccess) return nullptr;
+            c->sort_allocs.push_back(a);
+            return a;
+        };
+        c->sort_key[0] = static_cast<unsigned *>(alloc(sizeof(unsigned) * n));
+        c->sort_key[1] = static_cast<unsigned *>(alloc(sizeof(unsigned) * n));
+        c->sort_val = static_cast<int *>(alloc(sizeof(int) * n));
+        c->sort_idx = static_cast<int *>(alloc(sizeof(int) * n));
+        if (!c->sort_key[0] || !c->sort_key[1] || !c->sort_val || !c->sort_idx) {
+            c->sort_idx = nullptr;
+            return fail(EP_E_HIP, "hipMalloc of the re-sort buffers failed");
+        }
+#ifndef EP_CPU_CHECK
+        size_t bytes = 0;
+        HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, c->sort_key[0], c->sort_key[1], c->sort_val,
+                                                  c->sort_idx, int(n), 0, 32, s));
+        c->sort_tmp = alloc(bytes);
+        c->sort_tmp_bytes = bytes;
+        if (!c->sort_tmp) {
+            c->sort_idx = nullptr;
+            return fail(EP_E_HIP, "hipMalloc of the re-sort buffers failed");
+        }
+#endif
+    }
+    EP_LAUNCH((k_sort_keys<D, T>), gp, TB, s, c->ctrl, c->geom, P, n, int(c->brick[0]), int(c->brick[1]),
+              int(c->brick[2]), c->sort_key[0], c->sort_val);
+#ifdef EP_CPU_CHECK
+    {
+        // the checking build runs kernels on host threads, synchronously
+        std::vector<int> ord(static_cast<size_t>(n));
+        std::iota(ord.begin(), ord.end(), 0);
+        const unsigned *key = c->sort_key[0];
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return key[a] < key[b]; });
+        std::memcpy(c->sort_idx, ord.data(), sizeof(int) * size_t(n));
+    }
+#else
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(c->sort_tmp, c->sort_tmp_bytes, c->sort_key[0], c->sort_key[1],
+                                              c->sort_val, c->sort_idx, int(n), 0, 32, s));
+#endif
+    T *&spare = [&]() -> T *& {
+        if constexpr (std::is_same<T, double>::value) return c->spare_d;
+        else return c->spare_f;
+    }();
+    auto gather = [&](auto *&f, auto *&sp) {
+        if (!f) return;
+        using E = std::remove_pointer_t<std::remove_reference_t<decltype(f)>>;
+        EP_LAUNCH((k_permute<E>), gp, TB, s, f, sp, c->sort_idx, n);
+        std::swap(f, sp);
+    };
+    constexpr int DD = D * D, NB = D == 2 ? 3 : 6;
+    for (int k = 0; k < 2; k++)
+        for (int i = 0; i < D; i++) gather(P.x[k][i], c->spare_d);
+    for (int i = 0; i < D; i++) gather(P.v[i], spare);
+    for (int i = 0; i < DD; i++) gather(P.F[i], spare);
+    for (int i = 0; i < NB; i++) gather(P.b[i], spare);
+    for (int i = 0; i < NB; i++) gather(P.s[i], spare);
+    gather(P.epII, spare);
+    gather(P.dlam, spare);
+    gather(P.J, spare);
+    gather(P.rho, spare);
+    gather(P.Om0, spare);
+    gather(P.c0, spare);
+    gather(P.cr, spare);
+    gather(P.phi, spare);
+    for (int i = 0; i < D; i++) { gather(P.l[i], spare); gather(P.l0[i], spare); }
+    for (int i = 0; i < DD; i++) { gather(P.gv[i], spare); gather(P.Bm[i], spare); }
+    gather(c->perm_d, c->spare_i);
+    c->perm_dirty = true;
+    return EP_OK;
+}
+
+static int resort(ep_ctx *c) {
+    int r = EP_OK;
+    timed(c, K_SORT, [&] {
+        if (c->dim == 2) r = c->fp32 ? resort_t<2, float>(c, c->psf) : resort_t<2, double>(c, c->ps);
+        else r = c->fp32 ? resort_t<3, float>(c, c->psf) : resort_t<3, double>(c, c->ps);
+    });
+    return r;
+}
+
 static void dispatch_step(ep_ctx *c) {
     if (c->cfg.precision == EP_FP32_SURROGATE) {
         const int B = c->cfg.basis;
@@ -766,7 +919,15 @@ extern "C" int ep_step(ep_ctx *c, int32_t nsteps, const ep_step_params *par, ep_
     }
     HIPCHK(hipMemcpy(c->ctrl, &h, sizeof(Ctrl), hipMemcpyHostToDevice));
     c->ev_used = 0;
-    for (int k = 0; k < nsteps; k++) dispatch_step(c);
+    for (int k = 0; k < nsteps; k++) {
+        if (c->resort_every > 0 && !c->in_ensemble && c->since_sort >= c->resort_every) {
+            const int r = resort(c);
+            if (r != EP_OK) return r;
+            c->since_sort = 0;
+        }
+        dispatch_step(c);
+        c->since_sort++;
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     if (c->profile) {
@@ -1321,6 +1482,8 @@ extern "C" int ep_ensemble_create(ep_ctx *const *ctxs, int32_t n, ep_ensemble **
     if (c0.precision == EP_FP32_SURROGATE) return fail(EP_E_CONFIG, "ensembles run EP_FP64 or EP_FP32");
     ep_ensemble *e = new ep_ensemble();
     e->ctx.assign(ctxs, ctxs + n);
+    // the ensemble's instance table holds the contexts' field pointers
+    for (ep_ctx *c : e->ctx) c->in_ensemble = true;
     e->n = n;
     e->dim = c0.dim;
     e->basis = c0.basis;
@@ -1337,6 +1500,7 @@ extern "C" int ep_ensemble_create(ep_ctx *const *ctxs, int32_t n, ep_ensemble **
 
 extern "C" void ep_ensemble_destroy(ep_ensemble *e) {
     if (!e) return;
+    for (ep_ctx *c : e->ctx) c->in_ensemble = false;
     if (e->stream) hipStreamSynchronize(e->stream);
     ens_free_tables(e);
     hipFree(e->collect);

@@ -1066,6 +1066,49 @@ __device__ __forceinline__ void b_bij(const Ctrl *c, const Geom &g, const PState
 template <int D, int B, typename T>
 __global__ void __launch_bounds__(256) k_bij(const Ctrl *c, Geom g, PState<T> P, GState G) { b_bij<D, B, T>(c, g, P, G, blockIdx.x); }
 
+// Re-sort.  Particles drift out of the brick-major cell order the upload
+// gives them, and each block's node box grows with the spread of its
+// particles.  A re-sort recomputes every slot's key from its current
+// position (the upload's order: bricks of b0 x b1 x b2 cells, cells inside a
+// brick), sorts the slots stably by key (padding last) and gathers every
+// particle field through the sorted slot list.
+template <int D, typename T>
+__global__ void __launch_bounds__(256) k_sort_keys(const Ctrl *c, Geom g, PState<T> P, long nslot, int b0, int b1, int b2,
+                                                   unsigned *key, int *val) {
+    const long p = blockIdx.x * long(blockDim.x) + threadIdx.x;
+    if (p >= nslot) return;
+    unsigned k = 0xFFFFFFFFu;
+    if (P.Om0[p] != T(0)) {
+        long e[3] = {0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+            double q = exact_fld(P.x[c->parity][i][p] - g.x0[i], g.h[i], g.ih[i]);
+            if (!(q >= 0.0)) q = 0.0;
+            if (q > double(g.nel[i] - 1)) q = double(g.nel[i] - 1);
+            e[i] = long(q);
+        }
+        long kk;
+        if (D == 2) {
+            const long nbz = (g.nel[1] + b1 - 1) / b1;
+            const long brick = e[1] / b1 + nbz * (e[0] / b0);
+            kk = brick * (long(b0) * b1) + (e[1] % b1) + b1 * (e[0] % b0);
+        } else {
+            const long nbz = (g.nel[2] + b2 - 1) / b2, nbx = (g.nel[0] + b0 - 1) / b0;
+            const long brick = e[2] / b2 + nbz * (e[0] / b0 + nbx * (e[1] / b1));
+            kk = brick * (long(b0) * b1 * b2) + (e[2] % b2) + b2 * ((e[0] % b0) + b0 * (e[1] % b1));
+        }
+        k = unsigned(kk);
+    }
+    key[p] = k;
+    val[p] = int(p);
+}
+
+template <typename E>
+__global__ void __launch_bounds__(256) k_permute(const E *src, E *dst, const int *idx, long n) {
+    const long i = blockIdx.x * long(blockDim.x) + threadIdx.x;
+    if (i < n) dst[i] = src[idx[i]];
+}
+
 // augm_velocity (augm.jl:37-53) after mesh.v .= 0 (n2p.jl:67)
 template <int D>
 __device__ __forceinline__ void b_augv(const Ctrl *c, const GState &G, long blk) {

@@ -72,7 +72,7 @@ class EpStats(C.Structure):
 
 
 EXPORTS = ["ep_version", "ep_last_error", "ep_create", "ep_destroy", "ep_upload_grid", "ep_upload_particles",
-           "ep_download_particles", "ep_download_grid", "ep_set_time", "ep_step", "ep_get_dts", "ep_profile",
+           "ep_download_particles", "ep_download_grid", "ep_set_time", "ep_set_resort", "ep_step", "ep_get_dts", "ep_profile",
            "ep_kernel_times", "ep_get_stats", "ep_surrogate_create", "ep_surrogate_destroy",
            "ep_surrogate_get_params", "ep_surrogate_set_params", "ep_surrogate_sample", "ep_surrogate_get_samples",
            "ep_surrogate_train", "ep_surrogate_grad", "ep_surrogate_predict", "ep_attach_surrogate",
@@ -108,6 +108,7 @@ def load() -> C.CDLL:
     L.ep_download_particles.argtypes = [C.c_void_p, C.POINTER(EpParticlesView)]
     L.ep_download_grid.argtypes = [C.c_void_p, C.POINTER(EpGridFields)]
     L.ep_set_time.argtypes = [C.c_void_p, C.c_double]
+    L.ep_set_resort.argtypes = [C.c_void_p, C.c_int32]
     L.ep_step.argtypes = [C.c_void_p, C.c_int32, C.POINTER(EpStepParams), C.POINTER(EpStepResult)]
     L.ep_get_dts.argtypes = [C.c_void_p, _dp, C.c_int64, C.POINTER(C.c_int64)]
     L.ep_profile.argtypes = [C.c_void_p, C.c_int32]
@@ -217,6 +218,11 @@ class Device:
         _check(self.L.ep_upload_particles(self.h, C.byref(particles_view(self.mpts))))
         if t is not None:
             _check(self.L.ep_set_time(self.h, float(t)))
+
+    def set_resort(self, every: int):
+        """Re-sort the particles into brick-major cell order every `every`
+        steps (0: never)."""
+        _check(self.L.ep_set_resort(self.h, int(every)))
 
     def set_time(self, t: float):
         _check(self.L.ep_set_time(self.h, float(t)))

@@ -164,6 +164,13 @@ int ep_download_particles(ep_ctx *ctx, ep_particles_view *mp);
 int ep_download_grid(ep_ctx *ctx, ep_grid_fields *g);
 
 /* Time state: time.t[1] (and the step counter). */
+/* Keep the particles in brick-major cell order: every `every` steps of
+ * ep_step (0, the default: never) the device re-sorts the particles by the
+ * cell of their current position, so each block's node box stays compact as
+ * the particles move.  No counterpart in the reference (its particles keep
+ * their setup order); results change only in the summation order of the
+ * scatters.  Contexts in an ensemble are not re-sorted. */
+int ep_set_resort(ep_ctx *ctx, int32_t every);
 int ep_set_time(ep_ctx *ctx, double t);
 
 /* shpfun + mapsto + elasto/elastoplast for up to nsteps steps, with get_dt /

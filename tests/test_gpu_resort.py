@@ -1,0 +1,99 @@
+"""Periodic device re-sort (ep_set_resort): re-sorting the particles into
+brick-major cell order between steps changes nothing but the summation order
+of the scatters.  Every configuration is stepped with and without re-sorts
+and against the oracle, with downloads between ep_step calls (the slot list
+is read back from the device)."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+from cases import clone, compare, relerr, slump
+from elastoplasm import hip
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+EMU = "cpucheck" in os.environ.get("EP_HIP_LIB", "")
+
+CASES = [
+    dict(dim=3),
+    dict(dim=2, trsfr="tpic", musl=False),
+    dict(dim=2, trsfr="apic"),
+    dict(dim=2, basis="gimpm", how="Uii"),
+    dict(dim=2, basis="gimpm", deform="infinitesimal", how="ΔFii"),
+    dict(dim=3, precision="fp32"),
+]
+
+
+def _run(ic, cfg, chunks, every):
+    st = clone(ic)
+    dev = hip.Device(st.mesh, st.mpts, st.cmpr, cfg.instr)
+    dev.set_resort(every)
+    for n in chunks:
+        assert dev.step(n, 100.0).steps == n
+        dev.download()
+    dev.close()
+    return st
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "-".join(f"{k}={v}" for k, v in c.items()))
+def test_resort_changes_only_rounding(case):
+    case = dict(case)
+    prec = case.pop("precision", "fp64")
+    ic, cfg = slump(C_pf=1.0, **case)
+    cfg = clone(cfg)
+    cfg.instr["precision"] = prec
+    cfg.instr["ops"]["precision"] = {"fp64": 0, "fp32": 1}[prec]
+    chunks = (4, 3) if EMU else (13, 17)
+    a = _run(ic, cfg, chunks, 0)
+    b = _run(ic, cfg, chunks, 3)
+    fields = ["x", "v", "F", "J", "epII"] + (["tau", "b"] if case.get("deform", "finite") == "finite" else ["sig"])
+    if case.get("trsfr") == "tpic":
+        fields.append("gradv")
+    if case.get("trsfr") == "apic":
+        fields.append("Bij")
+    if case.get("basis") == "gimpm":
+        fields.append("l")
+    tol = 1e-11 if prec == "fp64" else 1e-4
+    err = compare(b.mpts, a.mpts, fields)
+    print("re-sorted vs not", err)
+    for f, e in err.items():
+        assert e <= tol, (f, e)
+    if prec == "fp64":
+        ref = clone(ic)
+        Oracle(ref.mesh, ref.mpts, ref.cmpr, cfg.instr).run(sum(chunks), 100.0)
+        x0 = ic.mpts.x
+        assert relerr(b.mpts.x - x0, ref.mpts.x - x0) <= 1e-9
+        assert max(compare(b.mpts, ref.mpts, fields).values()) <= 1e-9
+
+
+def test_resort_rejects_negative_and_skips_ensemble_members():
+    ic, cfg = slump(2)
+    dev = hip.Device(ic.mesh, ic.mpts, ic.cmpr, cfg.instr)
+    with pytest.raises(hip.EpError, match="EP_E_ARG"):
+        dev.set_resort(-1)
+    dev.close()
+    import elastoplasm as ep
+    lst, _ = ep.ic_ensemble(2, 7, scales=(1,), power=1.0, precision="fp64")
+    devs = []
+    for ic_i, cfg_i in lst:
+        st = clone(ic_i)
+        d = hip.Device(st.mesh, st.mpts, st.cmpr, cfg_i.instr)
+        d.set_resort(1)
+        devs.append((st, d))
+    ens = hip.Ensemble([d for _, d in devs])
+    ens.step(3, 100.0)
+    ens.download()
+    for (ic_i, cfg_i), (st, _) in zip(lst, devs):
+        alone = clone(ic_i)
+        d = hip.Device(alone.mesh, alone.mpts, alone.cmpr, cfg_i.instr)
+        d.step(3, 100.0)
+        d.download()
+        d.close()
+        assert relerr(st.mpts.s.tau, alone.mpts.s.tau) <= 1e-12
+    ens.close()
+    for _, d in devs:
+        d.close()
