@@ -194,7 +194,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the step would shard by spatial "
-              "slabs with halo exchange; see DESIGN.md).", file=sys.stderr)
+              "slabs with halo exchange).", file=sys.stderr)
         sys.exit(2)
 
     t_setup = time.perf_counter()

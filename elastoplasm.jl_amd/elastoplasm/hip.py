@@ -1,6 +1,6 @@
 """ctypes binding of libelastoplasm_hip.so (include/elastoplasm.h).
 
-This is the same binding a Julia ``ccall`` shim makes (INTEGRATION.md): raw
+This is the same binding a Julia ``ccall`` shim makes: raw
 pointers into the column-major host arrays of ``Mesh`` / ``Point``.  There is
 no CPU fallback: if the library is missing or no GPU is visible, the calls
 fail loudly.

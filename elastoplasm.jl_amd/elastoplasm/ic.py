@@ -159,7 +159,7 @@ def get_elastic_stiffness(E, nu, ndim):
 def setup_cmpr(mesh, E=1.0e6, nu=0.3, rho0=2700.0, **over) -> NT:
     """``setup_cmpr`` with the reference defaults.  The keyword overrides
     c0, cr, phi0, phir, Hp (hard-coded in the reference) are additions for
-    the BASELINE material sets."""
+    the cohesive column workloads."""
     K, G, Del = get_elastic_stiffness(E, nu, mesh.dim)
     c = math.sqrt((K + 4.0 / 3.0 * G) / rho0)
     c0, cr = 20.0e3, 4.0e3
@@ -227,8 +227,8 @@ def get_slump(mesh, cmpr, instr, ni=2, lz=12.80):
 
 def get_column(mesh, cmpr, instr, ni=2, width=0.25, height=None):
     """Rectangular column x <= width*L_x, z <= height (default the domain
-    height L_z, i.e. below the 3D mesh's 2h headroom): the BASELINE column
-    geometry, an addition to the reference's wedge."""
+    height L_z, i.e. below the 3D mesh's 2h headroom): the column geometry
+    of the 3D slump workloads, an addition to the reference's wedge."""
     x = _lattice(mesh, ni)
     keep = x[0, :] <= width * mesh.L[0]
     keep &= x[-1, :] <= (mesh.L[-1] if height is None else height)

@@ -85,7 +85,7 @@ def ic_slump(L, nel, fid: str = "slump", cmpr=None, geometry: str = "wedge", col
              **kwargs):
     """slump.jl:22-49.  ``cmpr`` (material overrides for setup_cmpr) and
     ``geometry`` ('wedge' = the reference's get_slump, 'column' = a
-    quarter-width column) are additions for the BASELINE material sets."""
+    quarter-width column) are additions for the cohesive column workloads."""
     instr = kwargser("instr", kwargs, dim=len(L))
     geom = get_geom(nel, L, instr)
     mesh = setup_mesh(instr, geom)

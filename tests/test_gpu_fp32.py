@@ -1,6 +1,6 @@
 """FP32 path (FP32 particle state and arithmetic, FP64 grid and
 near-degenerate eigen-decompositions) against the FP64 path and the oracle.
-Bar: relative stress error <= 1e-3 (BASELINE north star)."""
+Bar: relative stress error <= 1e-3."""
 from __future__ import annotations
 
 import math

@@ -1,6 +1,6 @@
 """Parity of the HIP path (through the C ABI) against the CPU oracle.
 
-Bar (BASELINE north star): plain FP64 matches the reference's CPU kernels to
+Bar: plain FP64 matches the reference's CPU kernels to
 <= 1e-10 relative per step, and <= 1e-6 relative in displacement and stress
 after 100 steps.  Errors are norm-wise over all particles,
 ||dev - ref|| / ||ref||.
@@ -21,7 +21,7 @@ pytestmark = pytest.mark.gpu
 STEP_TOL = 1e-10
 RUN_TOL = 1e-6
 
-# C1 materials (2D slump of the BASELINE config list)
+# C1 materials: the 2D slump with c = 20 kPa, φ = 30°
 C1 = dict(rho0=2650.0, E=1.0e6, nu=0.3, c0=20.0e3, cr=20.0e3, phi0=math.radians(30.0),
           phir=math.radians(30.0), Hp=0.0)
 
