@@ -204,3 +204,26 @@ def test_elastodynamic_ramp_parity():
     err = compare(dev.mpts, ref.mpts, ["x", "v", "F", "b", "tau"])
     print("elastodynamic", err)
     assert max(err.values()) <= RUN_TOL
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_resume_through_host_state(dim):
+    """slump! resumes from the mutated ic (slump.jl:103-109): here the state
+    goes to the host (ep_download_particles, vmax included), into a new
+    context (ep_upload_particles, which sorts by the new positions) and on
+    from time t (ep_set_time).  30 steps in one go and 15 + 15 across the
+    round trip agree to rounding, and both match the oracle."""
+    ic, cfg = slump(dim, C_pf=1.0, cmpr=C1)
+    one, _ = clone(ic), None
+    r1, _ = _device_run(one, cfg, 30, 100.0)
+    two = clone(ic)
+    ra, _ = _device_run(two, cfg, 15, 100.0)
+    rb, _ = _device_run(two, cfg, 15, 100.0, t0=ra.t)
+    assert rb.t == pytest.approx(r1.t, rel=1e-12)
+    fields = ["x", "v", "F", "J", "b", "tau", "epII"]
+    err = compare(two.mpts, one.mpts, fields)
+    print("resume vs one run", err)
+    assert max(err.values()) <= 1e-11
+    ref = clone(ic)
+    Oracle(ref.mesh, ref.mpts, ref.cmpr, cfg.instr).run(30, 100.0)
+    assert max(compare(two.mpts, ref.mpts, fields).values()) <= RUN_TOL
