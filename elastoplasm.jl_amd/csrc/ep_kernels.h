@@ -404,7 +404,7 @@ __device__ __forceinline__ void for_stencil_rot(const Geom &g, const W<D, T> &w,
             }
         }
     } else {
-        const int rz = lane & 3, rx = (lane >> 2) & 1;
+        const int rz = lane & 3, rx = (lane >> 2) & 3;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             if (!(w.ok >> (4 + k) & 1u)) continue;
@@ -444,10 +444,11 @@ __device__ __forceinline__ void for_stencil_rot(const Geom &g, const W<D, T> &w,
 template <int D, typename T, bool ROT, typename Fn>
 __device__ __forceinline__ void for_stencil_box(const BlockBox<D> &b, const W<D, T> &w, Fn &&fn) {
     // Scatter rotation (as for_stencil_rot): z by lane % 4 and, in 3D, x by
-    // (lane / 4) % 2, so the particles of one cell in consecutive lanes hit
-    // different nodes at each step.  Of the rotations measured this one was
-    // fastest: the LDS serves 16 lanes per pass and this leaves at most two
-    // of them on one bank pair.
+    // (lane / 4) % 4, so up to 16 particles of one cell in consecutive lanes
+    // hit different nodes at each step.  Measured on the 6.7M-particle 3D
+    // slump: p2g 2114 -> 1800 us averaged over 55 steps and 2322 -> 1883 us
+    // over 155 against x by (lane / 4) % 2 (cells fill beyond 8 particles as
+    // the column collapses); a rotation over all 64 nodes measured slower.
     const int lane = threadIdx.x & 63;
     const int rz = ROT ? (lane & 3) : 0;
     T wz[4], dwz[4];
@@ -460,22 +461,26 @@ __device__ __forceinline__ void for_stencil_box(const BlockBox<D> &b, const W<D,
         dwz[j0] = ROT ? sel4(w.dw[D - 1], j) : w.dw[D - 1][j0];
     }
     if (D == 2) {
+        const int rx = ROT ? ((lane >> 2) & 3) : 0;
         const int l0 = (w.base[1] - b.lo[1]) + b.ext[1] * (w.base[0] - b.lo[0]);
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
+        for (int i0 = 0; i0 < 4; i0++) {
+            const int i = ROT ? ((i0 + rx) & 3) : i0;
+            const T wx = ROT ? sel4(w.w[0], i) : w.w[0][i0];
+            const T dwx = ROT ? sel4(w.dw[0], i) : w.dw[0][i0];
             const int row = l0 + b.ext[1] * i;
 #pragma unroll
             for (int j0 = 0; j0 < 4; j0++) {
                 T dN[D];
-                const T N = w.w[0][i] * wz[j0];
-                dN[0] = w.dw[0][i] * wz[j0];
-                dN[1] = w.w[0][i] * dwz[j0];
+                const T N = wx * wz[j0];
+                dN[0] = dwx * wz[j0];
+                dN[1] = wx * dwz[j0];
                 const int ix[2] = {w.base[0] + i, w.base[1] + oz[j0]};
                 fn(0L, row + oz[j0], N, dN, ix);
             }
         }
     } else {
-        const int rx = ROT ? ((lane >> 2) & 1) : 0;
+        const int rx = ROT ? ((lane >> 2) & 3) : 0;
         T wx[4], dwx[4];
         int ox[4];
 #pragma unroll
