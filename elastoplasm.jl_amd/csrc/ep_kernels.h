@@ -443,12 +443,13 @@ __device__ __forceinline__ void for_stencil_rot(const Geom &g, const W<D, T> &w,
 // reference order, and the skipped nodes of for_stencil add exact zeros here.
 template <int D, typename T, bool ROT, typename Fn>
 __device__ __forceinline__ void for_stencil_box(const BlockBox<D> &b, const W<D, T> &w, Fn &&fn) {
-    // Scatter rotation (as for_stencil_rot): z by lane % 4 and, in 3D, x by
-    // (lane / 4) % 4, so up to 16 particles of one cell in consecutive lanes
-    // hit different nodes at each step.  Measured on the 6.7M-particle 3D
-    // slump: p2g 2114 -> 1800 us averaged over 55 steps and 2322 -> 1883 us
-    // over 155 against x by (lane / 4) % 2 (cells fill beyond 8 particles as
-    // the column collapses); a rotation over all 64 nodes measured slower.
+    // Scatter rotation: z by lane % 4, x by (lane / 4) % 4 and, in 3D, y by
+    // (lane / 16) % 4, so the particles of one cell in the lanes of a wave
+    // start their walks on 64 different nodes (cells hold 8 particles at
+    // rest and well over 16 as the column collapses).  Measured on the
+    // 6.7M-particle 3D slump over the bench's 55 steps, for 8 / 16 / 32 / 64
+    // start nodes: p2g 2114 / 1800 / 1641 / 1642 us, g2p 1545 / 1442 / 1316 /
+    // 1278 us.
     const int lane = threadIdx.x & 63;
     const int rz = ROT ? (lane & 3) : 0;
     T wz[4], dwz[4];
@@ -491,14 +492,18 @@ __device__ __forceinline__ void for_stencil_box(const BlockBox<D> &b, const W<D,
             dwx[i0] = ROT ? sel4(w.dw[0], i) : w.dw[0][i0];
         }
         const int l0 = (w.base[2] - b.lo[2]) + b.ext[2] * ((w.base[0] - b.lo[0]) + b.ext[0] * (w.base[1] - b.lo[1]));
+        const int ry = ROT ? ((lane >> 4) & 3) : 0;
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
+        for (int k0 = 0; k0 < 4; k0++) {
+            const int k = ROT ? ((k0 + ry) & 3) : k0;
+            const T wy = ROT ? sel4(w.w[1], k) : w.w[1][k0];
+            const T dwy = ROT ? sel4(w.dw[1], k) : w.dw[1][k0];
 #pragma unroll
             for (int i0 = 0; i0 < 4; i0++) {
                 const int row = l0 + b.ext[2] * (ox[i0] + b.ext[0] * k);
-                const T wxy = wx[i0] * w.w[1][k];
-                const T dxy = dwx[i0] * w.w[1][k];
-                const T xdy = wx[i0] * w.dw[1][k];
+                const T wxy = wx[i0] * wy;
+                const T dxy = dwx[i0] * wy;
+                const T xdy = wx[i0] * dwy;
 #pragma unroll
                 for (int j0 = 0; j0 < 4; j0++) {
                     T dN[D];
