@@ -53,6 +53,9 @@ inline thread_local EmuBlock emu_block;
 // A thread that has returned from the kernel drops out of later barriers, as
 // an exited wave does on the GPU.
 inline void __syncthreads() { emu_block.bar->arrive_and_wait(); }
+// device math the host library lacks
+inline double rsqrt(double x) { return 1.0 / std::sqrt(x); }
+inline float rsqrt(float x) { return 1.0f / std::sqrt(x); }
 
 inline void unsafeAtomicAdd(double *a, double v) { std::atomic_ref<double>(*a).fetch_add(v); }
 inline void unsafeAtomicAdd(float *a, float v) { std::atomic_ref<float>(*a).fetch_add(v); }

@@ -102,24 +102,26 @@ __device__ __forceinline__ void sym_eig(T (&a)[D][D], T (&v)[D][D]) {
                     continue;
                 }
                 rotated = true;
-                const T theta = (a[q][q] - a[p][p]) / (T(2) * apq);
-                T t;
-                if (fabs(theta) > T(sizeof(T) == 8 ? 1e150 : 1e18)) t = T(0.5) / theta;
-                else t = (theta >= T(0) ? T(1) : T(-1)) / (fabs(theta) + sqrt(theta * theta + T(1)));
-                const T c = T(1) / sqrt(t * t + T(1)), s = t * c;
-#pragma unroll
-                for (int k = 0; k < D; k++) {
-                    const T akp = a[k][p], akq = a[k][q];
-                    a[k][p] = c * akp - s * akq;
-                    a[k][q] = s * akp + c * akq;
-                }
-#pragma unroll
-                for (int k = 0; k < D; k++) {
-                    const T apk = a[p][k], aqk = a[q][k];
-                    a[p][k] = c * apk - s * aqk;
-                    a[q][k] = s * apk + c * aqk;
-                }
+                // t = tan of the rotation angle, the smaller root of
+                // t² + 2βt/a_pq - 1 = 0 with β = (a_qq - a_pp)/2; one divide,
+                // one square root and one reciprocal square root
+                const T beta = T(0.5) * (a[q][q] - a[p][p]);
+                const T r = sqrt(beta * beta + apq * apq);
+                const T t = apq / (beta >= T(0) ? beta + r : beta - r);
+                const T c = rsqrt(t * t + T(1)), s = t * c;
+                // symmetric two-sided update: the pivot pair in closed form,
+                // the other entries of rows/columns p and q once
+                a[p][p] -= t * apq;
+                a[q][q] += t * apq;
                 a[p][q] = a[q][p] = T(0);
+#pragma unroll
+                for (int k = 0; k < D; k++) {
+                    if (k == p || k == q) continue;
+                    const T akp = a[k][p], akq = a[k][q];
+                    const T np = c * akp - s * akq, nq = s * akp + c * akq;
+                    a[k][p] = a[p][k] = np;
+                    a[k][q] = a[q][k] = nq;
+                }
 #pragma unroll
                 for (int k = 0; k < D; k++) {
                     const T vkp = v[k][p], vkq = v[k][q];
