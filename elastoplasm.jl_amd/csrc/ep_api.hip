@@ -494,16 +494,16 @@ extern "C" int ep_upload_particles(ep_ctx *c, const ep_particles_view *v) {
     if (M.apic && !v->Bij) return fail(EP_E_ARG, "APIC needs Bij");
     for (long p = 0; p < n; p++)
         if (!(v->Om0[p] > 0.0)) return fail(EP_E_ARG, "particle volumes Om0 must be positive");
-    // cell-sorted order; in 3D every cell's run starts on a multiple of 8
-    // slots (padding in between), so the particles of a cell fill aligned
-    // 8-lane groups of a wave
+    // cell-sorted order.  Cell runs could be padded to a multiple of G slots
+    // (padding slots have Ω₀ = 0 and every kernel skips them); no kernel
+    // needs aligned runs, so G = 1.
     std::vector<long> key(n), order(n);
     for (long p = 0; p < n; p++) key[p] = brick_key(c, v->x, p);
     std::iota(order.begin(), order.end(), 0L);
     std::stable_sort(order.begin(), order.end(), [&](long a, long b) { return key[a] < key[b]; });
     c->perm.clear();
     c->donor.clear();
-    const long G = D == 3 ? 8 : 1;
+    const long G = 1;
     for (long a = 0; a < n;) {
         long b = a;
         while (b < n && key[order[b]] == key[order[a]]) b++;

@@ -776,115 +776,8 @@ __device__ __forceinline__ void b_p2g(const Ctrl *c, const Geom &g, const PState
         }
     });
     };
-    bool grp = false;
-    int b0 = 0, b1 = 0, b2 = 0, sub = 0;
-    // FP32 only: in FP64 the staging below (26 KiB) and the group sums'
-    // registers cut the kernel to 2 waves per SIMD, which costs more than
-    // the atomics save (measured 2.23 ms against 1.88 ms per launch at 6.7M
-    // particles)
-    constexpr bool kGrp = D == 3 && B == 0 && std::is_same<T, float>::value;
-    constexpr int NG = kGrp ? 256 : 1;
-    __shared__ double sx[3 * NG];
-    __shared__ T sv[10 * NG];
-    if constexpr (kGrp) {
-        // Cell groups.  The 8 lanes of an aligned lane group that hold 8 live
-        // particles of one cell (upload pads every cell to whole groups), all
-        // with fast-path weights, share the same 64 stencil nodes.  Each lane
-        // of such a group owns 8 of the nodes (y index k = (lane%8)/2, x
-        // index 2(lane%2) + {0, 1}, the 4 z), sums the contributions of the
-        // group's 8 particles in registers and adds them to the box: 56 LDS
-        // atomics per particle instead of 448.  The particles' data reach the
-        // other lanes through LDS.
-        const int lane = threadIdx.x & 63, g0 = lane & ~7;
-        sub = lane & 7;
-        b0 = __shfl(w.base[0], g0);
-        b1 = __shfl(w.base[1], g0);
-        b2 = __shfl(w.base[2], g0);
-        const bool cand = live && w.fast == 7u && w.base[0] == b0 && w.base[1] == b1 && w.base[2] == b2;
-        const unsigned long long gm = __ballot(cand);
-        grp = lds && !tpic && ((gm >> g0) & 0xFFull) == 0xFFull;
-        if (lds && !tpic) {
-            const int t0 = threadIdx.x;
-#pragma unroll
-            for (int i = 0; i < 3; i++) sx[3 * t0 + i] = xp[i];
-            sv[10 * t0 + 0] = ms;
-#pragma unroll
-            for (int d = 0; d < 3; d++) sv[10 * t0 + 1 + d] = pm[d];
-#pragma unroll
-            for (int k = 0; k < 6; k++) sv[10 * t0 + 4 + k] = Om * s[k];
-            __syncthreads();
-        }
-    }
-    // the per-particle walks first: the group sums below read only LDS, so
-    // none of the particle's registers stay live through them
-    if (lds) {
-        if (!grp) scatter(LdsSink<D, NV>{&box});
-    } else {
-        scatter(GlobSink{dst});
-    }
-    if constexpr (kGrp) {
-        if (grp) {
-            // cubic B-spline interior pieces by node index n of the stencil:
-            // f = ((c3 ξ + c2) ξ + c1) ξ + c0 (bsmpm.jl, type 3)
-            auto piece = [](int n, T xi, T ih, T &f, T &df) {
-                const T c3 = n == 0 ? T(-1.0 / 6.0) : (n == 1 ? T(0.5) : (n == 2 ? T(-0.5) : T(1.0 / 6.0)));
-                const T c2 = (n == 0 || n == 3) ? T(1) : T(-1);
-                const T c1 = n == 0 ? T(-2) : (n == 3 ? T(2) : T(0));
-                const T c0 = (n == 0 || n == 3) ? T(4.0 / 3.0) : T(2.0 / 3.0);
-                f = ((c3 * xi + c2) * xi + c1) * xi + c0;
-                df = ((T(3) * c3 * xi + T(2) * c2) * xi + c1) * ih;
-            };
-            const int k = sub >> 1, ia = 2 * (sub & 1);
-            const double Yk = g.X[1][b1 + k], Xi[2] = {g.X[0][b0 + ia], g.X[0][b0 + ia + 1]};
-            double Zj[4];
-#pragma unroll
-            for (int j = 0; j < 4; j++) Zj[j] = g.X[2][b2 + j];
-            const T ihx = T(g.ih[0]), ihy = T(g.ih[1]), ihz = T(g.ih[2]);
-            const LdsSink<D, NV> sink{&box};
-            const int l0 = (b2 - box.lo[2]) + box.ext[2] * ((b0 - box.lo[0]) + box.ext[0] * (b1 + k - box.lo[1]));
-            // one x column per pass keeps 28 accumulators live
-#pragma unroll 1
-            for (int a = 0; a < 2; a++) {
-                T acc[4][7];
-#pragma unroll
-                for (int j = 0; j < 4; j++)
-#pragma unroll
-                    for (int v = 0; v < 7; v++) acc[j][v] = T(0);
-#pragma unroll 2
-                for (int t = 0; t < 8; t++) {
-                    const int sl = (threadIdx.x & ~7) + t;
-                    T wy, dwy, wx, dwx, wz[4], dwz[4];
-                    piece(k, T((sx[3 * sl + 1] - Yk) * g.ih[1]), ihy, wy, dwy);
-                    piece(ia + a, T((sx[3 * sl] - Xi[a]) * g.ih[0]), ihx, wx, dwx);
-#pragma unroll
-                    for (int j = 0; j < 4; j++) piece(j, T((sx[3 * sl + 2] - Zj[j]) * g.ih[2]), ihz, wz[j], dwz[j]);
-                    const T tm = sv[10 * sl], tp0 = sv[10 * sl + 1], tp1 = sv[10 * sl + 2], tp2 = sv[10 * sl + 3];
-                    const T q0 = sv[10 * sl + 4], q1 = sv[10 * sl + 5], q2 = sv[10 * sl + 6];
-                    const T q3 = sv[10 * sl + 7], q4 = sv[10 * sl + 8], q5 = sv[10 * sl + 9];
-                    const T tg = tm * gz;
-                    const T wxy = wx * wy, dxy = dwx * wy, xdy = wx * dwy;
-#pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        const T N = wxy * wz[j], d0 = dxy * wz[j], d1 = xdy * wz[j], d2 = wxy * dwz[j];
-                        acc[j][0] += N * tm;
-                        acc[j][1] += N * tp0;
-                        acc[j][2] += N * tp1;
-                        acc[j][3] += N * tp2;
-                        // Voigt [xx,yy,zz,yz,xz,xy]
-                        acc[j][4] -= d0 * q0 + d1 * q5 + d2 * q4;
-                        acc[j][5] -= d0 * q5 + d1 * q1 + d2 * q3;
-                        acc[j][6] -= (d0 * q4 + d1 * q3 + d2 * q2) - N * tg;
-                    }
-                }
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const int li = l0 + j + box.ext[2] * (ia + a);
-#pragma unroll
-                    for (int v = 0; v < 7; v++) sink(v, 0L, li, double(acc[j][v]));
-                }
-            }
-        }
-    }
+    if (lds) scatter(LdsSink<D, NV>{&box});
+    else scatter(GlobSink{dst});
     if (lds) box.template flush<NV>(g, dst);
 }
 

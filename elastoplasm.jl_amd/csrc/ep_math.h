@@ -103,11 +103,13 @@ __device__ __forceinline__ void sym_eig(T (&a)[D][D], T (&v)[D][D]) {
                 }
                 rotated = true;
                 // t = tan of the rotation angle, the smaller root of
-                // t² + 2βt/a_pq - 1 = 0 with β = (a_qq - a_pp)/2; one divide,
-                // one square root and one reciprocal square root
-                const T beta = T(0.5) * (a[q][q] - a[p][p]);
-                const T r = sqrt(beta * beta + apq * apq);
-                const T t = apq / (beta >= T(0) ? beta + r : beta - r);
+                // t² + 2θt - 1 = 0 with θ = (a_qq - a_pp)/(2 a_pq).  θ is a
+                // ratio, so tiny entries (the FP32 path decomposes b - I) do
+                // not underflow as β² + a_pq² would
+                const T theta = (a[q][q] - a[p][p]) / (T(2) * apq);
+                T t;
+                if (fabs(theta) > T(sizeof(T) == 8 ? 1e150 : 1e18)) t = T(0.5) / theta;
+                else t = (theta >= T(0) ? T(1) : T(-1)) / (fabs(theta) + sqrt(theta * theta + T(1)));
                 const T c = rsqrt(t * t + T(1)), s = t * c;
                 // symmetric two-sided update: the pivot pair in closed form,
                 // the other entries of rows/columns p and q once
