@@ -99,12 +99,10 @@ struct W {
     T w[D][4], dw[D][4];
     int base[D];
     unsigned ok;    // bit (4*axis + k): node exists
-    unsigned fast;  // bit axis: four interior nodes, weights from the fixed pieces
     // A lane past the last particle: no node, zero weights, and a base
     // inside the block's node box so that box walks stay in bounds.
     __device__ __forceinline__ void retire(const int *lo) {
         ok = 0u;
-        fast = 0u;
 #pragma unroll
         for (int i = 0; i < D; i++) {
             base[i] = lo[i];
@@ -120,7 +118,6 @@ template <int D, int B, typename T>
 __device__ __forceinline__ bool weights(const Geom &g, const double *xp, const T *lp, W<D, T> &w) {
     bool good = true;
     w.ok = 0u;
-    w.fast = 0u;
 #pragma unroll
     for (int i = 0; i < D; i++) {
         const double a = xp[i] - g.x0[i];
@@ -154,7 +151,6 @@ __device__ __forceinline__ bool weights(const Geom &g, const double *xp, const T
                 w.w[i][3] = c16 * x3 + x2 + T(2) * xi[3] + c43;
                 w.dw[i][3] = (T(0.5) * x2 + T(2) * xi[3] + T(2)) * ih;
                 w.ok |= 0xFu << (4 * i);
-                w.fast |= 1u << i;
                 continue;
             }
         }
