@@ -255,8 +255,11 @@ __device__ __forceinline__ T mass_of(const PState<T> &P, const Mat &M, long p, T
 // the kernels fall back to global loads / global atomics.
 // -------------------------------------------------------------------------
 constexpr int ACC_LDS = 4096;   // doubles of LDS per block (32 KiB)
-// field v of a box with nv fields starts at v * lds_stride(nv): a constant, so
-// the fields of one node are immediate offsets of one LDS address
+// Box layout node-major: field v of box node k at k * nv + v, so the fields
+// of one node are adjacent, immediate offsets of one LDS address (measured
+// against field-major at the bench's 55 steps: g2p 1258 -> 1215 us, fbar
+// 1494 -> 1463 us, p2g and deform unchanged).  A box of nv fields holds up
+// to lds_stride(nv) nodes.
 __host__ __device__ constexpr int lds_stride(int nv) { return ACC_LDS / nv; }
 
 template <int D>
@@ -312,15 +315,13 @@ struct BlockBox {
         for (int k = threadIdx.x; k < nloc; k += blockDim.x) {
             const long no = node(g, k);
 #pragma unroll
-            for (int v = 0; v < NV; v++) buf[v * lds_stride(NV) + k] = no >= 0 ? src[v][no] : 0.0;
+            for (int v = 0; v < NV; v++) buf[k * NV + v] = no >= 0 ? src[v][no] : 0.0;
         }
         __syncthreads();
     }
     __device__ __forceinline__ void zero(int nv) {
         __syncthreads();   // finished reading staged values
-        const int st = lds_stride(nv);
-        for (int v = 0; v < nv; v++)
-            for (int k = threadIdx.x; k < nloc; k += blockDim.x) buf[v * st + k] = 0.0;
+        for (int k = threadIdx.x; k < nloc * nv; k += blockDim.x) buf[k] = 0.0;
         __syncthreads();
     }
     template <int NV>
@@ -331,7 +332,7 @@ struct BlockBox {
             if (no < 0) continue;
 #pragma unroll
             for (int v = 0; v < NV; v++) {
-                const double val = buf[v * lds_stride(NV) + k];
+                const double val = buf[k * NV + v];
                 if (val != 0.0) unsafeAtomicAdd(&dst[v][no], val);
             }
         }
@@ -344,7 +345,7 @@ template <int D, int NV>
 struct LdsSink {
     BlockBox<D> *b;
     __device__ __forceinline__ void operator()(int v, long, int li, double val) const {
-        lds_atomic_add(&b->buf[v * lds_stride(NV) + li], val);
+        lds_atomic_add(&b->buf[li * NV + v], val);
     }
 };
 struct GlobSink {
@@ -357,7 +358,7 @@ struct GlobSink {
 template <int D, int NV>
 struct LdsSrc {
     const BlockBox<D> *b;
-    __device__ __forceinline__ double operator()(int v, long, int li) const { return b->buf[v * lds_stride(NV) + li]; }
+    __device__ __forceinline__ double operator()(int v, long, int li) const { return b->buf[li * NV + v]; }
 };
 struct GlobSrc {
     const double *const *src;
@@ -1508,8 +1509,8 @@ __device__ __forceinline__ void b_fbar(Ctrl *c, const Geom &g, const PState<T> &
                 qv = m == 0.0 ? 0.0 : G.dJ[no] / m;
             }
 #pragma unroll
-            for (int d = 0; d < D; d++) box.buf[d * lds_stride(D + 1) + k] = no >= 0 ? G.v[d][no] : 0.0;
-            box.buf[D * lds_stride(D + 1) + k] = qv;
+            for (int d = 0; d < D; d++) box.buf[k * (D + 1) + d] = no >= 0 ? G.v[d][no] : 0.0;
+            box.buf[k * (D + 1) + D] = qv;
         }
         __syncthreads();
     }
@@ -1537,7 +1538,7 @@ __device__ __forceinline__ void b_fbar(Ctrl *c, const Geom &g, const PState<T> &
     const T rdiv = kDev<T> ? T(1) : dJp;
     if (staged) {
         gather_walk<D, T>(g, w, LdsSrc<D, D + 1>{&box}, [&](long, int li, T N, const T *, const int *) {
-            acc += N * T(box.buf[D * lds_stride(D + 1) + li]) / rdiv;
+            acc += N * T(box.buf[li * (D + 1) + D]) / rdiv;
         });
     } else {
         for_stencil<D, T>(g, w, [&](long no, T N, const T *, const int *) {
