@@ -200,7 +200,11 @@ int ep_grf_gauss(int32_t dim, int64_t n, const double *x, int32_t nh, const doub
  * blocks of all instances.  Each instance keeps its own dt (get_dt rule),
  * time window and state; ep_download_particles on an instance reads it back.
  * All instances share dim, the B-spline basis, transfer scheme, deformation
- * framework, plasticity and precision (EP_FP64 or EP_FP32). */
+ * framework, plasticity and precision (EP_FP64 or EP_FP32).  The ensemble
+ * holds its contexts' device buffers: destroy it before any of its contexts,
+ * and do not re-upload a member meanwhile (ep_ensemble_tune does its own
+ * re-uploads); members are not re-sorted (ep_set_resort) while they belong
+ * to an ensemble. */
 typedef struct ep_ensemble ep_ensemble;
 typedef struct ep_tune_result {
     int32_t block;            /* threads per particle / node block chosen */
