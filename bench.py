@@ -36,13 +36,15 @@ from elastoplasm import hip  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 SURR_TOL = 3e-5      # scaled residual a learned-return prediction must meet
-RESORT = 0           # steps between device re-sorts of the particles into cell order (0: never)
+RESORT = -20         # device re-sort policy: adaptive, checked every 20 steps (ep_set_resort)
 MAT = dict(rho0=2650.0, E=1.0e6, nu=0.3, c0=20.0e3, cr=20.0e3, phi0=math.radians(30.0), phir=math.radians(30.0),
            Hp=0.0)
 
 WORKLOADS = {
     # name: (L, nel, basis, column width)
     "slump3d_256x64x128": ([256.0, 64.0, 128.0], [256, 64, 128], "bsmpm", 0.4),
+    # the same domain filled completely: 16.8M particles
+    "block3d_256x64x128_full": ([256.0, 64.0, 128.0], [256, 64, 128], "bsmpm", 1.0),
     "slump3d_128x32x32_gimp": ([128.0, 32.0, 32.0], [128, 32, 32], "gimpm", 0.4),
     "slump2d_40x10": ([64.1584, 16.0396], [40, 10], "bsmpm", None),
 }
@@ -88,6 +90,16 @@ def build_workload(name: str, basis_override=None):
     return ic, cfg
 
 
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(name: str, budget_s: float, threads: int):
     """The CPU oracle (a restatement of the reference's KernelAbstractions CPU
     kernels, OpenMP over particles with atomic scatters) on a y-slab of the
@@ -124,8 +136,10 @@ def cpu_baseline(name: str, budget_s: float, threads: int):
 
 def ensemble_leg(steps: int) -> dict:
     """256 independent 2D slumps of uneven size (nel 40x10 .. 320x80, seed
-    43), FP32, cubic B-spline, TPIC-USL: autotuned layout, then `steps` graph
-    replays (one per step of every instance)."""
+    43; quarter-width columns), FP32, cubic B-spline, TPIC with the mUSL
+    re-projection: autotuned layout, then `steps` graph replays (one per step
+    of every instance).  A failure is reported in the leg and marks the line
+    non-finite."""
     t0 = time.perf_counter()
     lst, draw = ep.ic_ensemble(256, 43)
     devs = []
@@ -134,7 +148,10 @@ def ensemble_leg(steps: int) -> dict:
     e = hip.Ensemble(devs)
     nmp = sum(ic.mpts.nmp for ic, _ in lst)
     t_setup = time.perf_counter() - t0
-    scheme = "cubic B-spline, TPIC-USL, finite, F-bar, Drucker-Prager"
+    scheme = "cubic B-spline, TPIC + mUSL, finite, F-bar, Drucker-Prager, quarter-width columns"
+    head = {"instances": 256, "nmp_total": nmp, "nmp_min": int(min(ic.mpts.nmp for ic, _ in lst)),
+            "nmp_max": int(max(ic.mpts.nmp for ic, _ in lst)), "steps": steps, "unit": "particle-updates/s",
+            "dtype": "f32", "scheme": scheme}
     try:
         t0 = time.perf_counter()
         tune = e.tune(20, 1e9)
@@ -143,15 +160,13 @@ def ensemble_leg(steps: int) -> dict:
         t0 = time.perf_counter()
         r = e.step(steps, 1e9)
         el = time.perf_counter() - t0
-        assert all(x[2] == steps for x in r), "instances skipped steps"
+        if not all(x[2] == steps for x in r):
+            raise RuntimeError("instances skipped steps")
         e.download()
         finite = all(bool(np.isfinite(ic.mpts.s.tau).all()) for ic, _ in lst)
-    except hip.EpError as err:
-        # an instance that goes non-finite (EP_E_NONFINITE) is reported, not
-        # allowed to end the whole benchmark
-        print(f"[bench] ensemble: {err}", file=sys.stderr, flush=True)
-        return {"instances": 256, "nmp_total": nmp, "steps": steps, "value": None, "unit": "particle-updates/s",
-                "dtype": "f32", "scheme": scheme, "error": str(err), "state_finite": False}
+    except (hip.EpError, RuntimeError) as err:
+        print(f"[bench] ensemble FAILED: {err}", file=sys.stderr, flush=True)
+        return dict(head, value=None, error=str(err), state_finite=False)
     finally:
         e.close()
         for d in devs:
@@ -160,11 +175,7 @@ def ensemble_leg(steps: int) -> dict:
     print(f"[bench] ensemble: 256 slumps, {nmp} particles, {steps} steps in {el:.3f}s = {value:.4g} pu/s "
           f"({1000 * el / steps:.3f} ms/step); tuned block {tune['block']} brick {tune['brick']} "
           f"(setup {t_setup:.1f}s, tuning {t_tune:.1f}s)", file=sys.stderr, flush=True)
-    return {"instances": 256, "nmp_total": nmp, "nmp_min": int(min(ic.mpts.nmp for ic, _ in lst)),
-            "nmp_max": int(max(ic.mpts.nmp for ic, _ in lst)), "steps": steps, "value": value,
-            "unit": "particle-updates/s", "ms_per_step": 1000 * el / steps, "dtype": "f32",
-            "scheme": scheme, "autotune": tune,
-            "state_finite": finite}
+    return dict(head, value=value, ms_per_step=1000 * el / steps, autotune=tune, state_finite=finite)
 
 
 def measured_traffic(workload: str, precision: str, kernel: str):
@@ -190,16 +201,20 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="slump3d_256x64x128", choices=list(WORKLOADS))
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision", default="all", choices=["all", "fp64", "fp32", "fp32_surrogate"],
                     help="fp64 alone gives the headline line without the comparisons (used under profilers)")
     ap.add_argument("--surr-samples", type=int, default=10_000_000, help="training samples of the learned return")
     ap.add_argument("--surr-steps", type=int, default=6000, help="Adam steps (batch 65536)")
     ap.add_argument("--ensemble-steps", type=int, default=1000, help="steps of the 256-slump ensemble (0: skip)")
+    ap.add_argument("--long-steps", type=int, default=2000,
+                    help="whole-trajectory run of the workload, FP64 and FP32 + learned return (0: skip)")
+    ap.add_argument("--full-fill-steps", type=int, default=20,
+                    help="steps of the 16.8M-particle full-fill line (0: skip)")
     ap.add_argument("--surr-tol", type=float, default=SURR_TOL, help="residual tolerance of the learned return")
     ap.add_argument("--resort", type=int, default=RESORT,
-                    help="re-sort the particles into cell order every K steps (0: never)")
+                    help="re-sort policy (ep_set_resort): K > 0 every K steps, -K adaptive every K steps, 0 never")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the step would shard by spatial "
@@ -231,16 +246,20 @@ def main():
                 "sample_s": t_s, "train_s": t_t, "train_samples_per_s": args.surr_steps * 65536 / t_t,
                 "loss_first": float(loss[:10].mean()), "loss_last": float(loss[-50:].mean())}
 
-    def timed_run(precision):
-        """warmup + timed steps from the initial state; returns (pu/s, s,
-        per-kernel {name: ms per launch}, final state, stats)"""
-        st = copy.deepcopy(ic)
+    def device(st, precision):
         instr = copy.deepcopy(cfg.instr)
         instr["ops"]["precision"] = {"fp64": 0, "fp32": 1, "fp32_surrogate": 2}[precision]
         dev = hip.Device(st.mesh, st.mpts, st.cmpr, instr)
         dev.set_resort(args.resort)
         if precision == "fp32_surrogate":
             dev.attach_surrogate(net, args.surr_tol)
+        return dev
+
+    def timed_run(precision):
+        """warmup + timed steps from the initial state; returns (pu/s, s,
+        per-kernel {name: ms per launch}, final state, stats)"""
+        st = copy.deepcopy(ic)
+        dev = device(st, precision)
         if args.warmup:
             dev.step(args.warmup, 1e9)
         dev.profile(True)
@@ -254,6 +273,38 @@ def main():
         stats = dev.stats()
         dev.close()
         return nmp * args.steps / el, el, {k: ms / n for k, (ms, n) in kt.items()}, st, stats
+
+    def long_run(precision, steps, tail=100):
+        tail = max(1, min(tail, steps // 2))
+        """The whole trajectory from rest (C5 runs 2000 steps): wall time of
+        every step, per-kernel times of the last `tail` steps, re-sorts."""
+        st = copy.deepcopy(ic)
+        dev = device(st, precision)
+        t0 = time.perf_counter()
+        try:
+            dev.step(steps - tail, 1e9)
+            dev.profile(True)
+            r = dev.step(tail, 1e9)
+            el = time.perf_counter() - t0
+            kt = dev.kernel_times()
+            dev.download()
+            stats = dev.stats()
+        except hip.EpError as err:
+            print(f"[bench] long run {precision} FAILED: {err}", file=sys.stderr, flush=True)
+            return {"precision": precision, "steps": steps, "value": None, "error": str(err), "state_finite": False}
+        finally:
+            dev.close()
+        finite = bool(np.isfinite(st.mpts.s.tau).all() and np.isfinite(st.mpts.x).all())
+        inside = bool(((st.mpts.x >= ic.mesh.xB[:, :1]) & (st.mpts.x <= ic.mesh.xB[:, 1:])).all())
+        out = {"precision": precision, "steps": steps, "value": nmp * steps / el, "ms_per_step": 1000 * el / steps,
+               "t_end": r.t, "resorts": int(stats.resorts), "state_finite": finite, "particles_inside_domain": inside,
+               "last_steps": tail, "kernels_us_last_steps": {k: 1000 * ms / n for k, (ms, n) in kt.items()}}
+        if precision == "fp32_surrogate":
+            out.update(predictions=int(stats.predictions), accepted=int(stats.predictions - stats.fallbacks))
+        print(f"[bench] long run {precision}: {steps} steps in {el:.2f}s = {out['value']:.4g} pu/s "
+              f"({out['ms_per_step']:.3f} ms/step), {out['resorts']} re-sorts, p2g at the end "
+              f"{out['kernels_us_last_steps'].get('p2g', float('nan')):.0f} us", file=sys.stderr, flush=True)
+        return out
 
     value, el, per_kernel, st64, _ = timed_run("fp64")
     ms_step = 1000.0 * el / args.steps
@@ -272,10 +323,11 @@ def main():
             "algorithmic_bytes": kb * nmp}
     step_roof = {"bytes_per_particle": B_ALG[dim], "achieved": B_ALG[dim] * value / 1e9,
                  "frac": B_ALG[dim] * value / 1e9 / HBM_PEAK_GBS, "kernel_ms_per_step": step_ms}
-    finite = bool(np.isfinite(st64.mpts.s.tau).all())
+    finite = {"fp64": bool(np.isfinite(st64.mpts.s.tau).all())}
     fp32 = None
     if args.precision in ("all", "fp32"):
         v32, el32, pk32, st32, _ = timed_run("fp32")
+        finite["fp32"] = bool(np.isfinite(st32.mpts.s.tau).all())
         err32 = float(np.linalg.norm(st32.mpts.s.tau - st64.mpts.s.tau) / np.linalg.norm(st64.mpts.s.tau))
         x0 = ic.mpts.x
         errx32 = float(np.linalg.norm(st32.mpts.x - st64.mpts.x) / np.linalg.norm(st64.mpts.x - x0))
@@ -293,8 +345,10 @@ def main():
                 "rel_stress_err_vs_fp64": err32, "rel_displacement_err_vs_fp64": errx32,
                 "kernels_us": {k: v * 1000 for k, v in pk32.items()},
                 "dtype": "f32 particles, f64 grid and near-degenerate eigen-decompositions"}
+        del st32
     if net is not None:
         vs, els, pks, sts, stt = timed_run("fp32_surrogate")
+        finite["fp32_surrogate"] = bool(np.isfinite(sts.mpts.s.tau).all())
         errs = float(np.linalg.norm(sts.mpts.s.tau - st64.mpts.s.tau) / np.linalg.norm(st64.mpts.s.tau))
         x0 = ic.mpts.x
         errxs = float(np.linalg.norm(sts.mpts.x - st64.mpts.x) / np.linalg.norm(st64.mpts.x - x0))
@@ -309,14 +363,34 @@ def main():
                      "fallback_fraction": (int(stt.fallbacks) / pred) if pred else None,
                      "max_scaled_residual": float(stt.max_residual), "tol": args.surr_tol,
                      "kernels_us": {k: v * 1000 for k, v in pks.items()}})
+        del sts
+    del st64
+    longr = None
+    if args.long_steps > 0 and args.precision == "all":
+        longr = {"fp64": long_run("fp64", args.long_steps)}
+        if net is not None:
+            longr["fp32_surrogate"] = long_run("fp32_surrogate", args.long_steps)
+        for k, v in longr.items():
+            finite[f"long_{k}"] = v["state_finite"]
+    if net is not None:
         net.close()
     ens = None
     if args.ensemble_steps > 0 and args.precision == "all":
         ens = ensemble_leg(args.ensemble_steps)
+        finite["ensemble"] = ens["state_finite"]
+    full = None
+    if args.full_fill_steps > 0 and args.precision == "all" and args.workload == "slump3d_256x64x128":
+        full = full_fill_leg(args.full_fill_steps, args.warmup, args.resort)
+        finite["full_fill"] = full["state_finite"]
     cpu = None
     if not args.no_cpu_baseline:
         thr = min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(args.workload, args.cpu_seconds, thr)
+        one = cpu_baseline(args.workload, args.cpu_seconds, 1)
+        cpu.update({"one_core": {"value": one["value"], "cores": 1, "sample": one["sample"]},
+                    "nproc": os.cpu_count(), "cpu_model": _cpu_model()})
+        print(f"[bench] cpu oracle: {cpu['value']:.4g} pu/s on {thr} threads, {one['value']:.4g} on 1 "
+              f"({cpu['cpu_model']}, nproc {cpu['nproc']})", file=sys.stderr, flush=True)
     out = {
         "metric": "particle-updates/s on 1 MI355X (plain FP64 explicit MPM step)",
         "value": value,
@@ -334,17 +408,56 @@ def main():
                    "transfer": "mUSL-FLIP", "deform": "finite", "locking": True, "plast": "DP",
                    "cohesion": (f"log-normal random field, mean {GRF['mean']:g} Pa, CoV {GRF['cov']}, "
                                 f"correlation length {GRF['ell']} m, seed {GRF['seed']}" if dim == 3 else "uniform"),
-                   "precision": "fp64", "resort_every": args.resort, "parallelism": "single-gpu"},
+                   "precision": "fp64", "resort": args.resort, "parallelism": "single-gpu"},
         "roofline": roof,
         "step_roofline": step_roof,
         "kernels_us": {k: v * 1000 for k, v in per_kernel.items()},
         "fp32": fp32,
         "fp32_surrogate": surr,
+        "long_run": longr,
         "ensemble": ens,
-        "state_finite": finite,
+        "full_fill": full,
+        "state_finite": all(finite.values()),
+        "state_finite_legs": finite,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def full_fill_leg(steps: int, warmup: int, resort: int) -> dict:
+    """The 256 x 64 x 128 m domain filled with 2x2x2 particles per cell
+    (16.8M particles), the same scheme and materials, FP64 and FP32: the
+    particle count of the 16M-particle target on one GPU."""
+    t0 = time.perf_counter()
+    ic, cfg = build_workload("block3d_256x64x128_full")
+    nmp = ic.mpts.nmp
+    t_setup = time.perf_counter() - t0
+    out = {"workload": "block3d_256x64x128_full", "nmp": nmp, "steps": steps, "warmup": warmup,
+           "unit": "particle-updates/s", "setup_s": t_setup}
+    finite = True
+    for prec in ("fp64", "fp32"):
+        instr = copy.deepcopy(cfg.instr)
+        instr["ops"]["precision"] = {"fp64": 0, "fp32": 1}[prec]
+        dev = hip.Device(ic.mesh, ic.mpts, ic.cmpr, instr)   # uploads the initial state; never downloaded
+        dev.set_resort(resort)
+        try:
+            dev.step(warmup, 1e9)
+            dev.profile(True)
+            t0 = time.perf_counter()
+            dev.step(steps, 1e9)
+            el = time.perf_counter() - t0
+            kt = dev.kernel_times()
+            out[prec] = {"value": nmp * steps / el, "ms_per_step": 1000 * el / steps,
+                         "kernels_us": {k: 1000 * ms / n for k, (ms, n) in kt.items()}}
+        except hip.EpError as err:
+            out[prec] = {"value": None, "error": str(err)}
+            finite = False
+        finally:
+            dev.close()
+        print(f"[bench] full fill {prec}: {nmp} particles, {out[prec].get('ms_per_step', float('nan')):.3f} ms/step, "
+              f"{out[prec]['value'] or 0:.4g} pu/s", file=sys.stderr, flush=True)
+    out["state_finite"] = finite
+    return out
 
 
 if __name__ == "__main__":

@@ -43,6 +43,12 @@ enum KernelId { K_BEGIN, K_ZERO, K_P2G, K_EULER, K_G2P, K_BIJ, K_AUGV, K_DEFORM,
 const char *kKernelNames[K_COUNT] = {"begin", "zero_grid", "p2g", "euler", "g2p", "bij", "augv", "deform", "fbar",
                                      "surrogate", "end", "resort"};
 
+// Default re-sort policy of a context: adaptive, checking the p2g box
+// overflow count every 20 steps and re-sorting when more than 0.2 % of the
+// blocks of those steps fell back to global atomics.
+constexpr int kResortAdaptive = -20;
+constexpr double kSpillFrac = 0.002;
+
 // LU with partial pivoting of Del (same algorithm the device solve expects).
 void factor_del(int n, const double *A, DelLU &f) {
     std::memcpy(f.LU, A, sizeof(double) * n * n);
@@ -97,8 +103,12 @@ struct ep_ctx {
     bool part_ready = false;
     // periodic re-sort into brick-major cell order (k_sort_keys): slot list
     // on the device, spare field buffers of each element type, sort scratch
-    int resort_every = 0;
+    // resort_every > 0: every that many steps; < 0: adaptive, the p2g box
+    // overflow count is read every -resort_every steps; 0: never
+    int resort_every = kResortAdaptive;
     long since_sort = 0;
+    int spill_seen = 0;                // ctrl->spill at the last check
+    long resorts = 0;
     bool in_ensemble = false;          // an ensemble holds this context's field pointers
     int *perm_d = nullptr;             // device copy of perm, permuted with the fields
     bool perm_dirty = false;
@@ -563,7 +573,6 @@ extern "C" int ep_upload_particles(ep_ctx *c, const ep_particles_view *v) {
 
 extern "C" int ep_set_resort(ep_ctx *c, int32_t every) {
     if (!c) return fail(EP_E_ARG, "null context");
-    if (every < 0) return fail(EP_E_ARG, "ep_set_resort: every must be >= 0");
     c->resort_every = every;
     c->since_sort = 0;
     return EP_OK;
@@ -902,6 +911,7 @@ extern "C" int ep_step(ep_ctx *c, int32_t nsteps, const ep_step_params *par, ep_
     h.dt_fixed = par->dt_fixed;
     h.record = par->record_dt;
     h.steps = 0;
+    h.spill = 0;
     h.err = 0;
     h.j2cap = 0;
     h.s_acc = 0;
@@ -919,10 +929,26 @@ extern "C" int ep_step(ep_ctx *c, int32_t nsteps, const ep_step_params *par, ep_
     }
     HIPCHK(hipMemcpy(c->ctrl, &h, sizeof(Ctrl), hipMemcpyHostToDevice));
     c->ev_used = 0;
+    c->spill_seen = 0;
     for (int k = 0; k < nsteps; k++) {
-        if (c->resort_every > 0 && !c->in_ensemble && c->since_sort >= c->resort_every) {
-            const int r = resort(c);
-            if (r != EP_OK) return r;
+        const int every = c->resort_every;
+        if (every != 0 && !c->in_ensemble && c->since_sort >= (every > 0 ? every : -every)) {
+            bool go = every > 0;
+            if (!go) {
+                // adaptive: re-sort once more than kSpillFrac of the p2g blocks
+                // of the window fell back to global atomics
+                int spill = 0;
+                HIPCHK(hipMemcpyAsync(&spill, &c->ctrl->spill, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+                HIPCHK(hipStreamSynchronize(c->stream));
+                const long nblk = (c->nslot + c->tb - 1) / c->tb;
+                go = double(spill - c->spill_seen) > kSpillFrac * double(nblk) * double(c->since_sort);
+                c->spill_seen = spill;
+            }
+            if (go) {
+                const int r = resort(c);
+                if (r != EP_OK) return r;
+                c->resorts++;
+            }
             c->since_sort = 0;
         }
         dispatch_step(c);
@@ -997,6 +1023,7 @@ extern "C" int ep_kernel_times(ep_ctx *c, int32_t max_kernels, char *names, int3
 extern "C" int ep_get_stats(ep_ctx *c, ep_stats *st) {
     if (!c || !st) return fail(EP_E_ARG, "null argument");
     *st = c->stats;
+    st->resorts = c->resorts;
     return EP_OK;
 }
 

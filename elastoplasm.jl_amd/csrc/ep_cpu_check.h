@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <barrier>
 #include <chrono>
+#include <condition_variable>
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -32,7 +33,15 @@ using std::isnan;
 #define __host__
 #define __forceinline__ inline
 #define __launch_bounds__(...)
+#ifdef EP_CPU_FIBERS
+// Fiber variant (-DEP_CPU_FIBERS, no AddressSanitizer): the threads of a block
+// are fibers of one host thread that switch at __syncthreads, and blocks run
+// in parallel on a pool of host threads, so block-local memory is per host
+// thread.  Fast enough for whole trajectories of long runs on the CPU.
+#define __shared__ static thread_local
+#else
 #define __shared__ static
+#endif
 
 struct dim3 {
     unsigned x, y, z;
@@ -52,7 +61,12 @@ inline thread_local EmuBlock emu_block;
 
 // A thread that has returned from the kernel drops out of later barriers, as
 // an exited wave does on the GPU.
+#ifdef EP_CPU_FIBERS
+inline void emu_fiber_yield();
+inline void __syncthreads() { emu_fiber_yield(); }
+#else
 inline void __syncthreads() { emu_block.bar->arrive_and_wait(); }
+#endif
 // device math the host library lacks
 inline double rsqrt(double x) { return 1.0 / std::sqrt(x); }
 inline float rsqrt(float x) { return 1.0f / std::sqrt(x); }
@@ -333,6 +347,201 @@ inline void emu_launch(void (*k)(P...), dim3 g, dim3 b, A... args) {
     emu_launch_now(k, g, b, args...);
 }
 
+#ifdef EP_CPU_FIBERS
+// ---- fibers: a block's threads on one host thread ----
+// emu_ctx_swap saves the callee-saved registers on the current stack, stores
+// the stack pointer in *save and resumes the stack `load` (x86-64 SysV).
+extern "C" void emu_ctx_swap(void **save, void *load);
+asm(R"(
+    .text
+    .globl emu_ctx_swap
+    .type emu_ctx_swap, @function
+emu_ctx_swap:
+    pushq %rbp
+    pushq %rbx
+    pushq %r12
+    pushq %r13
+    pushq %r14
+    pushq %r15
+    movq %rsp, (%rdi)
+    movq %rsi, %rsp
+    popq %r15
+    popq %r14
+    popq %r13
+    popq %r12
+    popq %rbx
+    popq %rbp
+    ret
+    .size emu_ctx_swap, .-emu_ctx_swap
+)");
+
+struct EmuFiberBlock {
+    static constexpr size_t kStack = 512 * 1024;
+    unsigned n = 0;
+    std::vector<std::unique_ptr<char[]>> stack;
+    std::vector<void *> sp;
+    void *sched = nullptr;
+    unsigned cur = 0;
+    const std::function<void(unsigned)> *job = nullptr;
+    std::vector<unsigned long long> shfl, mf_a, mf_b;
+    std::vector<unsigned char> alive;
+    std::vector<unsigned> order;
+    void size(unsigned n_) {
+        if (n_ == n) return;
+        n = n_;
+        stack.clear();
+        for (unsigned t = 0; t < n; t++) stack.emplace_back(new char[kStack]);
+        sp.assign(n, nullptr);
+        shfl.assign(n, 0);
+        mf_a.assign(n, 0);
+        mf_b.assign(n, 0);
+        alive.assign(n, 0);
+        order.resize(n);
+    }
+};
+inline thread_local EmuFiberBlock *emu_fb = nullptr;
+
+[[noreturn]] inline void emu_fiber_entry() {
+    EmuFiberBlock *f = emu_fb;
+    const unsigned t = f->cur;
+    (*f->job)(t);
+    f->alive[t] = 0;
+    void *dead;
+    emu_ctx_swap(&dead, f->sched);
+    std::abort();   // never resumed
+}
+
+inline void emu_fiber_yield() {
+    EmuFiberBlock *f = emu_fb;
+    emu_ctx_swap(&f->sp[f->cur], f->sched);
+}
+
+// Run one block: every live fiber runs to its next __syncthreads (or its
+// end) in each pass, in the order `order`, so a pass is one barrier phase.
+inline void emu_fiber_run(EmuFiberBlock &f, const std::function<void(unsigned)> &job, unsigned long long seed) {
+    f.job = &job;
+    for (unsigned t = 0; t < f.n; t++) {
+        // initial frame: six zero registers, then emu_fiber_entry as the return address
+        char *top = f.stack[t].get() + EmuFiberBlock::kStack;
+        top = reinterpret_cast<char *>(reinterpret_cast<uintptr_t>(top) & ~uintptr_t(15));
+        void **s = reinterpret_cast<void **>(top);
+        *--s = nullptr;
+        *--s = reinterpret_cast<void *>(&emu_fiber_entry);
+        for (int r = 0; r < 6; r++) *--s = nullptr;
+        f.sp[t] = s;
+        f.alive[t] = 1;
+        f.order[t] = t;
+    }
+    if (seed) {
+        // a seeded permutation of the fibers: another order of block-local atomics
+        unsigned long long z = seed;
+        for (unsigned t = f.n - 1; t > 0; t--) {
+            z += 0x9E3779B97F4A7C15ull;
+            unsigned long long y = z;
+            y = (y ^ (y >> 30)) * 0xBF58476D1CE4E5B9ull;
+            y = (y ^ (y >> 27)) * 0x94D049BB133111EBull;
+            y ^= y >> 31;
+            std::swap(f.order[t], f.order[unsigned(y % (t + 1))]);
+        }
+    }
+    emu_fb = &f;
+    bool any = true;
+    while (any) {
+        any = false;
+        for (unsigned k = 0; k < f.n; k++) {
+            const unsigned t = f.order[k];
+            if (!f.alive[t]) continue;
+            f.cur = t;
+            threadIdx = {t, 0, 0};
+            emu_block = {nullptr, f.shfl.data(), f.alive.data(), f.mf_a.data(), f.mf_b.data()};
+            emu_ctx_swap(&f.sched, f.sp[t]);
+            if (f.alive[t]) any = true;
+        }
+    }
+}
+
+// Host-thread pool: the blocks of a launch are shared out dynamically.
+struct EmuPool {
+    unsigned nw;
+    std::vector<std::thread> th;
+    std::mutex mu;
+    std::condition_variable cv, cv_done;
+    unsigned long long gen = 0;
+    unsigned busy = 0;
+    bool quit = false;
+    std::function<void(unsigned)> work;   // argument: worker id
+    explicit EmuPool(unsigned n) : nw(n) {
+        for (unsigned w = 1; w < nw; w++)
+            th.emplace_back([this, w] {
+                unsigned long long seen = 0;
+                for (;;) {
+                    std::unique_lock<std::mutex> lk(mu);
+                    cv.wait(lk, [&] { return quit || gen != seen; });
+                    if (quit) return;
+                    seen = gen;
+                    lk.unlock();
+                    work(w);
+                    lk.lock();
+                    if (--busy == 0) cv_done.notify_all();
+                }
+            });
+    }
+    ~EmuPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            quit = true;
+        }
+        cv.notify_all();
+        for (auto &x : th) x.join();
+    }
+    void run(std::function<void(unsigned)> f) {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            work = std::move(f);
+            busy = nw - 1;
+            gen++;
+        }
+        cv.notify_all();
+        work(0);
+        std::unique_lock<std::mutex> lk(mu);
+        cv_done.wait(lk, [&] { return busy == 0; });
+    }
+};
+inline EmuPool &emu_pool() {
+    static EmuPool p([] {
+        const char *e = std::getenv("EP_EMU_THREADS");
+        const int n = e ? std::atoi(e) : int(std::thread::hardware_concurrency());
+        return unsigned(n < 1 ? 1 : n);
+    }());
+    return p;
+}
+inline std::atomic<unsigned long long> emu_launch_count{0};
+
+template <class... P, class... A>
+inline void emu_launch_now(void (*k)(P...), dim3 g, dim3 b, A... args) {
+    static thread_local EmuFiberBlock fb;
+    // EP_EMU_SEED: non-zero permutes each block's fiber order (a different
+    // order of block-local atomics per block and launch)
+    static const unsigned long long seed0 = [] {
+        const char *e = std::getenv("EP_EMU_SEED");
+        return e ? std::strtoull(e, nullptr, 10) : 0ull;
+    }();
+    const unsigned long long launch = emu_launch_count.fetch_add(1);
+    std::atomic<unsigned> next{0};
+    emu_pool().run([&](unsigned) {
+        fb.size(b.x);
+        for (;;) {
+            const unsigned bx = next.fetch_add(1);
+            if (bx >= g.x) break;
+            blockIdx = {bx, 0, 0};
+            blockDim = {b.x, 1, 1};
+            gridDim = {g.x, 1, 1};
+            const unsigned long long s = seed0 ? (seed0 * 0x100000001B3ull) ^ (launch << 24) ^ bx : 0ull;
+            emu_fiber_run(fb, [&](unsigned) { k(args...); }, s);
+        }
+    });
+}
+#else
 template <class... P, class... A>
 inline void emu_launch_now(void (*k)(P...), dim3 g, dim3 b, A... args) {
     EmuTeam &team = emu_team(b.x);
@@ -346,4 +555,5 @@ inline void emu_launch_now(void (*k)(P...), dim3 g, dim3 b, A... args) {
         });
     }
 }
+#endif
 #define EP_LAUNCH(kern, grid, block, stream, ...) emu_launch(kern, dim3(grid), dim3(block), __VA_ARGS__)

@@ -65,6 +65,7 @@ struct Ctrl {
     long dt_cap;
     unsigned long long *vmax_part;  // [3][nblk] per-block maxima of |v_p| written by G2P
     int nblk;
+    int spill;                      // particle blocks of p2g whose node box did not fit LDS (cumulative)
     // learned return mapping (FP32 + surrogate path)
     int qcount;                     // plastic trial states queued this step
     unsigned long long s_acc, s_fb; // predictions accepted / replaced by the FP64 return
@@ -255,17 +256,23 @@ __device__ __forceinline__ T mass_of(const PState<T> &P, const Mat &M, long p, T
 // the kernels fall back to global loads / global atomics.
 // -------------------------------------------------------------------------
 constexpr int ACC_LDS = 4096;   // doubles of LDS per block (32 KiB)
-// Box layout node-major: field v of box node k at k * nv + v, so the fields
-// of one node are adjacent, immediate offsets of one LDS address (measured
-// against field-major at the bench's 55 steps: g2p 1258 -> 1215 us, fbar
-// 1494 -> 1463 us, p2g and deform unchanged).  A box of nv fields holds up
-// to lds_stride(nv) nodes.
-__host__ __device__ constexpr int lds_stride(int nv) { return ACC_LDS / nv; }
-
+// Box layout.  The LDS serves a 64-bit access as four quarter-waves of 16
+// lanes, each through 32 four-byte banks, i.e. 16 bank pairs per double; the
+// measured bank-conflict counts of the walks match that model exactly.
+// A box node at local (z, x, y) keeps field v of an NV-field box at
+//   16*NV*G + 16*v + (z & 3) + 4*(x & 3),   G = z/4 + nz4*(x/4 + nx4*y),
+// so its bank pair is (z & 3) + 4*(x & 3) whatever the field.  Scatter walks
+// start each lane at the node with z = lane (mod 4), x = lane/4 (mod 4)
+// (for_stencil_box): the 16 lanes of a quarter-wave then hit 16 different
+// bank pairs, hence 16 different nodes, at every step of the walk for any
+// spread of particles over cells.  Fields of one node sit 16 doubles apart,
+// immediate offsets of one address.
 template <int D>
 struct BlockBox {
     int lo[D], ext[D];
     int nloc;
+    int nz4, nx4, ey;          // box extent in groups of 4 along z and x; y extent (1 in 2D)
+    bool any;                  // the block has a live particle
     EP_LDS double *buf;
 
     __device__ __forceinline__ void begin(EP_LDS double *smem, int *sint, bool live, const int *base) {
@@ -287,52 +294,66 @@ struct BlockBox {
             ext[i] = ok ? int(e) : 4;
             n *= ext[i];
         }
+        any = sint[0] != 0x7fffffff;
         nloc = (ok && n <= ACC_LDS) ? int(n) : 0;
+        nz4 = (ext[D - 1] + 3) >> 2;
+        nx4 = (ext[0] + 3) >> 2;
+        ey = D == 3 ? ext[1] : 1;
         __syncthreads();   // everyone read sint before a later begin() rewrites it
     }
-    __device__ __forceinline__ bool fits(int nv) const { return nloc > 0 && nloc <= lds_stride(nv); }
-    // box entries run as the grid: last axis (z) fastest, then x, then y
-    // global node of local box entry k, or -1 outside the grid
-    __device__ __forceinline__ long node(const Geom &g, int k) const {
-        int ix[3];
-        if (D == 2) {
-            ix[1] = lo[1] + k % ext[1];
-            ix[0] = lo[0] + k / ext[1];
-            if (ix[0] < 0 || ix[1] < 0 || ix[0] >= g.nno[0] || ix[1] >= g.nno[1]) return -1;
-            return long(ix[1]) + long(g.nno[1]) * ix[0];
-        }
-        ix[2] = lo[2] + k % ext[2];
-        const int r = k / ext[2];
-        ix[0] = lo[0] + r % ext[0];
-        ix[1] = lo[1] + r / ext[0];
-        if (ix[0] < 0 || ix[1] < 0 || ix[2] < 0 || ix[0] >= g.nno[0] || ix[1] >= g.nno[1] || ix[2] >= g.nno[2])
-            return -1;
-        return long(ix[2]) + long(g.nno[2]) * (long(ix[0]) + long(g.nno[0]) * ix[1]);
+    // doubles one field group of the layout spans (NV fields: NV times this)
+    __device__ __forceinline__ int span() const { return 16 * nz4 * nx4 * ey; }
+    __device__ __forceinline__ bool fits(int nv) const { return nloc > 0 && long(span()) * nv <= ACC_LDS; }
+    // address of field 0 of local node (z, x, y) in an NV-field box
+    template <int NV>
+    __device__ __forceinline__ int addr(int z, int x, int y) const {
+        return 16 * NV * ((z >> 2) + nz4 * ((x >> 2) + nx4 * y)) + (z & 3) + 4 * (x & 3);
     }
-    // LDS[v][k] = src[v][node(k)] (0 outside the grid); caller checks fits(nv)
+    // box entries run as the grid: last axis (z) fastest, then x, then y;
+    // local coordinates of entry k and its global node, or -1 outside the grid
+    __device__ __forceinline__ long node(const Geom &g, int k, int &z, int &x, int &y) const {
+        z = k % ext[D - 1];
+        const int r = k / ext[D - 1];
+        x = D == 2 ? r : r % ext[0];
+        y = D == 2 ? 0 : r / ext[0];
+        if (D == 2) {
+            const int ix0 = lo[0] + x, ix1 = lo[1] + z;
+            if (ix0 < 0 || ix1 < 0 || ix0 >= g.nno[0] || ix1 >= g.nno[1]) return -1;
+            return long(ix1) + long(g.nno[1]) * ix0;
+        }
+        const int ix0 = lo[0] + x, ix1 = lo[1] + y, ix2 = lo[2] + z;
+        if (ix0 < 0 || ix1 < 0 || ix2 < 0 || ix0 >= g.nno[0] || ix1 >= g.nno[1] || ix2 >= g.nno[2]) return -1;
+        return long(ix2) + long(g.nno[2]) * (long(ix0) + long(g.nno[0]) * ix1);
+    }
+    // LDS field v of every box node = src[v][node] (0 outside the grid); caller checks fits(NV)
     template <int NV>
     __device__ __forceinline__ void stage(const Geom &g, const double *const *src) {
         for (int k = threadIdx.x; k < nloc; k += blockDim.x) {
-            const long no = node(g, k);
+            int z, x, y;
+            const long no = node(g, k, z, x, y);
+            const int a = addr<NV>(z, x, y);
 #pragma unroll
-            for (int v = 0; v < NV; v++) buf[k * NV + v] = no >= 0 ? src[v][no] : 0.0;
+            for (int v = 0; v < NV; v++) buf[a + 16 * v] = no >= 0 ? src[v][no] : 0.0;
         }
         __syncthreads();
     }
     __device__ __forceinline__ void zero(int nv) {
         __syncthreads();   // finished reading staged values
-        for (int k = threadIdx.x; k < nloc * nv; k += blockDim.x) buf[k] = 0.0;
+        const int n = span() * nv;
+        for (int k = threadIdx.x; k < n; k += blockDim.x) buf[k] = 0.0;
         __syncthreads();
     }
     template <int NV>
     __device__ __forceinline__ void flush(const Geom &g, double *const *dst) {
         __syncthreads();
         for (int k = threadIdx.x; k < nloc; k += blockDim.x) {
-            const long no = node(g, k);
+            int z, x, y;
+            const long no = node(g, k, z, x, y);
             if (no < 0) continue;
+            const int a = addr<NV>(z, x, y);
 #pragma unroll
             for (int v = 0; v < NV; v++) {
-                const double val = buf[k * NV + v];
+                const double val = buf[a + 16 * v];
                 if (val != 0.0) unsafeAtomicAdd(&dst[v][no], val);
             }
         }
@@ -345,7 +366,7 @@ template <int D, int NV>
 struct LdsSink {
     BlockBox<D> *b;
     __device__ __forceinline__ void operator()(int v, long, int li, double val) const {
-        lds_atomic_add(&b->buf[li * NV + v], val);
+        lds_atomic_add(&b->buf[li + 16 * v], val);
     }
 };
 struct GlobSink {
@@ -358,7 +379,7 @@ struct GlobSink {
 template <int D, int NV>
 struct LdsSrc {
     const BlockBox<D> *b;
-    __device__ __forceinline__ double operator()(int v, long, int li) const { return b->buf[li * NV + v]; }
+    __device__ __forceinline__ double operator()(int v, long, int li) const { return b->buf[li + 16 * v]; }
 };
 struct GlobSrc {
     const double *const *src;
@@ -434,73 +455,66 @@ __device__ __forceinline__ void for_stencil_rot(const Geom &g, const W<D, T> &w,
 
 // Stencil walk over a block's LDS node box: no existence tests (nodes
 // outside the grid have zero weights, and their box slots are never
-// flushed), the box index of the particle's first node computed once, and
-// for scatters (ROT) the lane rotation applied to the weight arrays once per
-// particle instead of per node.  Without ROT the order is for_stencil's
-// reference order, and the skipped nodes of for_stencil add exact zeros here.
-template <int D, typename T, bool ROT, typename Fn>
+// flushed), the box address of each visited node from per-axis parts
+// computed once per particle.  Scatters (ROT) start lane l's walk at the node
+// whose local z is l (mod 4) and local x is l/4 (mod 4), and y at offset
+// (l/16) mod 4: with the box layout above, the 16 lanes of a quarter-wave
+// touch 16 different LDS bank pairs at every step, whatever cells their
+// particles lie in.  Without ROT the order is for_stencil's reference order,
+// and the skipped nodes of for_stencil add exact zeros here.  fn(node, box
+// address of field 0, N, dN, axis indices); field v is 16*v further.
+template <int D, typename T, bool ROT, int NV, typename Fn>
 __device__ __forceinline__ void for_stencil_box(const BlockBox<D> &b, const W<D, T> &w, Fn &&fn) {
-    // Scatter rotation: z by lane % 4, x by (lane / 4) % 4 and, in 3D, y by
-    // (lane / 16) % 4, so the particles of one cell in the lanes of a wave
-    // start their walks on 64 different nodes (cells hold 8 particles at
-    // rest and well over 16 as the column collapses).  Measured on the
-    // 6.7M-particle 3D slump over the bench's 55 steps, for 8 / 16 / 32 / 64
-    // start nodes: p2g 2114 / 1800 / 1641 / 1642 us, g2p 1545 / 1442 / 1316 /
-    // 1278 us.
     const int lane = threadIdx.x & 63;
-    const int rz = ROT ? (lane & 3) : 0;
-    T wz[4], dwz[4];
-    int oz[4];
+    const int zb = w.base[D - 1] - b.lo[D - 1], xb = w.base[0] - b.lo[0];
+    const int yb = D == 3 ? w.base[1] - b.lo[1] : 0;
+    const int rz = ROT ? ((lane - zb) & 3) : 0;
+    const int rx = ROT ? (((lane >> 2) - xb) & 3) : 0;
+    const int ry = ROT ? ((lane >> 4) & 3) : 0;
+    T wz[4], dwz[4], wx[4], dwx[4];
+    int oz[4], ox[4], az[4], ax[4];
 #pragma unroll
     for (int j0 = 0; j0 < 4; j0++) {
         const int j = (j0 + rz) & 3;
         oz[j0] = j;
         wz[j0] = ROT ? sel4(w.w[D - 1], j) : w.w[D - 1][j0];
         dwz[j0] = ROT ? sel4(w.dw[D - 1], j) : w.dw[D - 1][j0];
+        const int z = zb + j;
+        az[j0] = 16 * NV * (z >> 2) + (z & 3);
+        const int i = (j0 + rx) & 3;
+        ox[j0] = i;
+        wx[j0] = ROT ? sel4(w.w[0], i) : w.w[0][j0];
+        dwx[j0] = ROT ? sel4(w.dw[0], i) : w.dw[0][j0];
+        const int x = xb + i;
+        ax[j0] = 16 * NV * b.nz4 * (x >> 2) + 4 * (x & 3);
     }
     if (D == 2) {
-        const int rx = ROT ? ((lane >> 2) & 3) : 0;
-        const int l0 = (w.base[1] - b.lo[1]) + b.ext[1] * (w.base[0] - b.lo[0]);
 #pragma unroll
         for (int i0 = 0; i0 < 4; i0++) {
-            const int i = ROT ? ((i0 + rx) & 3) : i0;
-            const T wx = ROT ? sel4(w.w[0], i) : w.w[0][i0];
-            const T dwx = ROT ? sel4(w.dw[0], i) : w.dw[0][i0];
-            const int row = l0 + b.ext[1] * i;
 #pragma unroll
             for (int j0 = 0; j0 < 4; j0++) {
                 T dN[D];
-                const T N = wx * wz[j0];
-                dN[0] = dwx * wz[j0];
-                dN[1] = wx * dwz[j0];
-                const int ix[2] = {w.base[0] + i, w.base[1] + oz[j0]};
-                fn(0L, row + oz[j0], N, dN, ix);
+                const T N = wx[i0] * wz[j0];
+                dN[0] = dwx[i0] * wz[j0];
+                dN[1] = wx[i0] * dwz[j0];
+                const int ix[2] = {w.base[0] + ox[i0], w.base[1] + oz[j0]};
+                fn(0L, ax[i0] + az[j0], N, dN, ix);
             }
         }
     } else {
-        const int rx = ROT ? ((lane >> 2) & 3) : 0;
-        T wx[4], dwx[4];
-        int ox[4];
-#pragma unroll
-        for (int i0 = 0; i0 < 4; i0++) {
-            const int i = (i0 + rx) & 3;
-            ox[i0] = i;
-            wx[i0] = ROT ? sel4(w.w[0], i) : w.w[0][i0];
-            dwx[i0] = ROT ? sel4(w.dw[0], i) : w.dw[0][i0];
-        }
-        const int l0 = (w.base[2] - b.lo[2]) + b.ext[2] * ((w.base[0] - b.lo[0]) + b.ext[0] * (w.base[1] - b.lo[1]));
-        const int ry = ROT ? ((lane >> 4) & 3) : 0;
+        const int sy = 16 * NV * b.nz4 * b.nx4;
 #pragma unroll
         for (int k0 = 0; k0 < 4; k0++) {
             const int k = ROT ? ((k0 + ry) & 3) : k0;
             const T wy = ROT ? sel4(w.w[1], k) : w.w[1][k0];
             const T dwy = ROT ? sel4(w.dw[1], k) : w.dw[1][k0];
+            const int ay = sy * (yb + k);
 #pragma unroll
             for (int i0 = 0; i0 < 4; i0++) {
-                const int row = l0 + b.ext[2] * (ox[i0] + b.ext[0] * k);
                 const T wxy = wx[i0] * wy;
                 const T dxy = dwx[i0] * wy;
                 const T xdy = wx[i0] * dwy;
+                const int row = ay + ax[i0];
 #pragma unroll
                 for (int j0 = 0; j0 < 4; j0++) {
                     T dN[D];
@@ -509,7 +523,7 @@ __device__ __forceinline__ void for_stencil_box(const BlockBox<D> &b, const W<D,
                     dN[1] = xdy * wz[j0];
                     dN[2] = wxy * dwz[j0];
                     const int ix[3] = {w.base[0] + ox[i0], w.base[1] + k, w.base[2] + oz[j0]};
-                    fn(0L, row + oz[j0], N, dN, ix);
+                    fn(0L, row + az[j0], N, dN, ix);
                 }
             }
         }
@@ -521,7 +535,7 @@ __device__ __forceinline__ void for_stencil_box(const BlockBox<D> &b, const W<D,
 // (zero weight): clamp them before reading node coordinates.
 template <int D, typename T, int NV, typename Fn>
 __device__ __forceinline__ void scatter_walk(const Geom &, const W<D, T> &w, const LdsSink<D, NV> &s, Fn &&fn) {
-    for_stencil_box<D, T, true>(*s.b, w, fn);
+    for_stencil_box<D, T, true, NV>(*s.b, w, fn);
 }
 template <int D, typename T, typename Fn>
 __device__ __forceinline__ void scatter_walk(const Geom &g, const W<D, T> &w, const GlobSink &, Fn &&fn) {
@@ -529,7 +543,7 @@ __device__ __forceinline__ void scatter_walk(const Geom &g, const W<D, T> &w, co
 }
 template <int D, typename T, int NV, typename Fn>
 __device__ __forceinline__ void gather_walk(const Geom &, const W<D, T> &w, const LdsSrc<D, NV> &s, Fn &&fn) {
-    for_stencil_box<D, T, false>(*s.b, w, fn);
+    for_stencil_box<D, T, false, NV>(*s.b, w, fn);
 }
 template <int D, typename T, typename Fn>
 __device__ __forceinline__ void gather_walk(const Geom &g, const W<D, T> &w, const GlobSrc &, Fn &&fn) {
@@ -714,6 +728,7 @@ __device__ __forceinline__ void b_p2g(const Ctrl *c, const Geom &g, const PState
     if (!live) w.retire(box.lo);
     const bool lds = box.fits(NV);
     if (lds) box.zero(NV);
+    else if (threadIdx.x == 0 && box.any) atomicAdd(const_cast<int *>(&c->spill), 1);
     double *dst[NV];
     dst[0] = G.m;
 #pragma unroll
@@ -1502,15 +1517,17 @@ __device__ __forceinline__ void b_fbar(Ctrl *c, const Geom &g, const PState<T> &
     const bool staged = box.fits(D + 1);
     if (staged) {
         for (int k = threadIdx.x; k < box.nloc; k += blockDim.x) {
-            const long no = box.node(g, k);
+            int z, x, y;
+            const long no = box.node(g, k, z, x, y);
+            const int a = box.template addr<D + 1>(z, x, y);
             double qv = 0.0;
             if (no >= 0) {
                 const double m = G.m[no];
                 qv = m == 0.0 ? 0.0 : G.dJ[no] / m;
             }
 #pragma unroll
-            for (int d = 0; d < D; d++) box.buf[k * (D + 1) + d] = no >= 0 ? G.v[d][no] : 0.0;
-            box.buf[k * (D + 1) + D] = qv;
+            for (int d = 0; d < D; d++) box.buf[a + 16 * d] = no >= 0 ? G.v[d][no] : 0.0;
+            box.buf[a + 16 * D] = qv;
         }
         __syncthreads();
     }
@@ -1538,7 +1555,7 @@ __device__ __forceinline__ void b_fbar(Ctrl *c, const Geom &g, const PState<T> &
     const T rdiv = kDev<T> ? T(1) : dJp;
     if (staged) {
         gather_walk<D, T>(g, w, LdsSrc<D, D + 1>{&box}, [&](long, int li, T N, const T *, const int *) {
-            acc += N * T(box.buf[li * (D + 1) + D]) / rdiv;
+            acc += N * T(box.buf[li + 16 * D]) / rdiv;
         });
     } else {
         for_stencil<D, T>(g, w, [&](long no, T N, const T *, const int *) {

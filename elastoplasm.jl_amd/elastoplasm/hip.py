@@ -68,7 +68,8 @@ class EpTuneResult(C.Structure):
 
 class EpStats(C.Structure):
     _fields_ = [("launches", C.c_int64), ("steps", C.c_int64), ("fallbacks", C.c_int64),
-                ("predictions", C.c_int64), ("j2_maxit", C.c_int64), ("max_residual", C.c_double)]
+                ("predictions", C.c_int64), ("j2_maxit", C.c_int64), ("max_residual", C.c_double),
+                ("resorts", C.c_int64)]
 
 
 EXPORTS = ["ep_version", "ep_last_error", "ep_create", "ep_destroy", "ep_upload_grid", "ep_upload_particles",
@@ -221,7 +222,8 @@ class Device:
 
     def set_resort(self, every: int):
         """Re-sort the particles into brick-major cell order every `every`
-        steps (0: never)."""
+        steps; every < 0 (the library's default, -20): adaptively, checking
+        the P2G box overflow count every -every steps; 0: never."""
         _check(self.L.ep_set_resort(self.h, int(every)))
 
     def set_time(self, t: float):

@@ -107,15 +107,21 @@ def ic_slump(L, nel, fid: str = "slump", cmpr=None, geometry: str = "wedge", col
 ENSEMBLE_L = [64.1584, 16.0396]
 
 
-def ic_ensemble(n: int = 256, seed: int = 43, *, scales=(1, 2, 3, 4, 5, 6, 7, 8), power: float = 2.5,
-                trsfr: str = "tpic", musl: bool = False, precision: str = "fp32", **kwargs):
-    """An ensemble of n independent 2D slumps (the ic_slump wedge, 2x2
-    particles per cell) of uneven size: instance i has nel = (40 s, 10 s)
-    with s drawn from `scales` with probability ∝ s^-power (about 0.5M
-    particles over 256 instances), cohesion c log-uniform in [5, 50] kPa and
+def ic_ensemble(n: int = 256, seed: int = 43, *, scales=(1, 2, 3, 4, 5, 6, 7, 8), power: float = 2.0,
+                trsfr: str = "tpic", musl: bool = True, precision: str = "fp32", geometry: str = "column",
+                **kwargs):
+    """An ensemble of n independent 2D slumps (2x2 particles per cell) of
+    uneven size: instance i has nel = (40 s, 10 s) with s drawn from `scales`
+    with probability ∝ s^-power, cohesion c log-uniform in [5, 50] kPa and
     friction angle φ uniform in [20°, 40°] (no softening), ρ 2650, E 1e6,
-    ν 0.3; TPIC-USL (trsfr="tpic", musl=false) by default.  Returns the list
-    of (ic, cfg) pairs and the drawn (s, c, φ) per instance."""
+    ν 0.3.  Defaults: the quarter-width full-height column (about 0.54M
+    particles over 256 instances at power 2) and TPIC with the mUSL
+    re-projection (trsfr="tpic", musl=true, the TPIC form the reference
+    constructs, mapsto.jl:44-53).  Without mUSL (TPIC-USL) the finest
+    instances dilate without bound at free-surface particles, in the FP64
+    CPU oracle as on the device.  geometry="wedge" gives ic_slump's
+    get_slump geometry.  Returns the list of (ic, cfg) pairs and the drawn
+    (s, c, φ) per instance."""
     rng = np.random.default_rng(seed)
     sc = np.asarray(scales, dtype=np.int64)
     pr = sc.astype(np.float64) ** (-power)
@@ -130,7 +136,7 @@ def ic_ensemble(n: int = 256, seed: int = 43, *, scales=(1, 2, 3, 4, 5, 6, 7, 8)
                   fwrk=dict(deform="finite", trsfr=trsfr, musl=musl, C_pf=1.0, locking=True, damping=0.1),
                   nonloc=dict(status=False, ls=0.5), plast=dict(status=True, constitutive="DP"))
         kw.update(kwargs)
-        ic, cfg = ic_slump(ENSEMBLE_L, [40 * int(s[i]), 10 * int(s[i])], cmpr=mat, **kw)
+        ic, cfg = ic_slump(ENSEMBLE_L, [40 * int(s[i]), 10 * int(s[i])], cmpr=mat, geometry=geometry, **kw)
         cfg.instr["precision"] = precision
         cfg.instr["ops"]["precision"] = {"fp64": 0, "fp32": 1}[precision]
         out.append((ic, cfg))

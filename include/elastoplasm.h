@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define EP_ABI_VERSION 2   /* 2: ep_particles_view.Bij, ep_config.domain */
+#define EP_ABI_VERSION 3   /* 2: ep_particles_view.Bij, ep_config.domain; 3: ep_stats.resorts, adaptive re-sort */
 
 enum ep_status {
     EP_OK = 0,
@@ -144,6 +144,7 @@ typedef struct ep_stats {
     int64_t predictions;  /* surrogate predictions made */
     int64_t j2_maxit;     /* J2 particles that hit the iteration cap */
     double max_residual;  /* largest surrogate yield residual seen */
+    int64_t resorts;      /* device re-sorts of the particles (ep_set_resort) */
 } ep_stats;
 
 typedef struct ep_ctx ep_ctx;
@@ -163,14 +164,17 @@ int ep_upload_particles(ep_ctx *ctx, const ep_particles_view *mp);
 int ep_download_particles(ep_ctx *ctx, ep_particles_view *mp);
 int ep_download_grid(ep_ctx *ctx, ep_grid_fields *g);
 
-/* Time state: time.t[1] (and the step counter). */
-/* Keep the particles in brick-major cell order: every `every` steps of
- * ep_step (0, the default: never) the device re-sorts the particles by the
- * cell of their current position, so each block's node box stays compact as
- * the particles move.  No counterpart in the reference (its particles keep
- * their setup order); results change only in the summation order of the
- * scatters.  Contexts in an ensemble are not re-sorted. */
+/* Keep the particles in brick-major cell order: the device re-sorts the
+ * particles by the cell of their current position, so each block's node box
+ * stays compact as the particles move.  every > 0: every `every` steps of
+ * ep_step; every < 0 (the default, -20): adaptive, every -every steps the
+ * host reads the device count of P2G blocks whose node box overflowed LDS
+ * and re-sorts when they exceed 0.2 % of the blocks of those steps; 0:
+ * never.  No counterpart in the reference (its particles keep their setup
+ * order); results change only in the summation order of the scatters.
+ * Contexts in an ensemble are not re-sorted. */
 int ep_set_resort(ep_ctx *ctx, int32_t every);
+/* Time state: time.t[1] (and the step counter). */
 int ep_set_time(ep_ctx *ctx, double t);
 
 /* shpfun + mapsto + elasto/elastoplast for up to nsteps steps, with get_dt /

@@ -113,3 +113,54 @@ def test_slump3d_128x32x32_500_steps_fp32(plast, deform):
     for m in (a, b):
         assert np.isfinite(getattr(m.s, field)).all() and np.isfinite(m.x).all()
     assert npl >= 0.5 * a.nmp
+
+
+@pytest.mark.skipif(EMU, reason="418k particles: for the GPU")
+@pytest.mark.timeout(900)
+def test_slump3d_128x32x32_fp64_100_steps():
+    """FP64 against the oracle over 100 steps: <= 1e-6 in stress and in
+    displacement x - x0."""
+    ic, cfg = build("DP", "finite")
+    ref, d = clone(ic), clone(ic)
+    Oracle(ref.mesh, ref.mpts, ref.cmpr, cfg.instr, threads=min(16, os.cpu_count() or 1)).run(100, 100.0)
+    dev = hip.Device(d.mesh, d.mpts, d.cmpr, cfg.instr)
+    r = dev.step(100, 100.0)
+    dev.download()
+    dev.close()
+    assert r.steps == 100
+    x0 = ic.mpts.x
+    es, eu = relerr(d.mpts.s.tau, ref.mpts.s.tau), relerr(d.mpts.x - x0, ref.mpts.x - x0)
+    print(f"C2 DP finite 100 steps: stress {es:.2e}, displacement {eu:.2e}, "
+          f"plastic particles {(ref.mpts.s.epII[0] > 0).sum()}")
+    assert es <= 1e-6 and eu <= 1e-6
+
+
+@pytest.mark.skipif(EMU, reason="418k particles: for the GPU")
+def test_slump3d_128x32x32_learned_return():
+    """C3's inference inside C2: the learned Drucker-Prager return, trained
+    on the device, with the residual check and FP64 fallback, in the
+    418k-particle GIMP slump; within 1e-3 of FP64 over 20 steps."""
+    ic, cfg = build("DP", "finite")
+    net = hip.Surrogate(seed=45)
+    net.sample(2_000_000, 44)
+    net.train(2000, 65536, 2e-3, 2e-5)
+    out = {}
+    for prec in ("fp64", "fp32_surrogate"):
+        d = clone(ic)
+        c = clone(cfg)
+        c.instr["precision"] = prec
+        c.instr["ops"]["precision"] = {"fp64": 0, "fp32_surrogate": 2}[prec]
+        dev = hip.Device(d.mesh, d.mpts, d.cmpr, c.instr)
+        if prec == "fp32_surrogate":
+            dev.attach_surrogate(net, 3e-5)
+        assert dev.step(20, 100.0).steps == 20
+        dev.download()
+        out[prec] = (d, dev.stats())
+        dev.close()
+    net.close()
+    a, b = out["fp64"][0].mpts, out["fp32_surrogate"][0].mpts
+    st = out["fp32_surrogate"][1]
+    es = relerr(b.s.tau, a.s.tau)
+    print(f"C2 learned return 20 steps: stress err vs fp64 {es:.2e}; {st.predictions} predictions, "
+          f"{st.predictions - st.fallbacks} accepted")
+    assert st.predictions > 0 and es <= 1e-3

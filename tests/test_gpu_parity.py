@@ -131,10 +131,43 @@ def test_one_step_2d_j2():
     _one_step(ic, cfg, 40, ["x", "v", "F", "J", "b", "tau", "epII"])
 
 
-@pytest.mark.parametrize("trsfr", ["std", "apic"])
-def test_one_step_3d(trsfr):
-    ic, cfg = slump(3, trsfr=trsfr)
-    _one_step(ic, cfg, 10, ["x", "v", "F", "J", "b", "tau", "epII", "dlam"] + (["Bij"] if trsfr == "apic" else []))
+# the reference's 3D slump test matrix (test_slump.jl:3-16, 66-73): bsmpm,
+# musl = true, C_pf = 0.99, every deformation x transfer x locking row
+CASES_3D = [dict(deform=d, trsfr=t, locking=lk) for d in ("finite", "infinitesimal") for t in ("std", "tpic", "apic")
+            for lk in (True, False)]
+
+
+@pytest.mark.parametrize("case", CASES_3D, ids=lambda c: "-".join(f"{k}={v}" for k, v in c.items()))
+def test_one_step_3d(case):
+    ic, cfg = slump(3, musl=True, **case)
+    fields = ["x", "v", "F", "J", "epII", "dlam"] + (["b", "tau"] if case["deform"] == "finite" else ["sig"])
+    if case["trsfr"] == "tpic":
+        fields.append("gradv")
+    if case["trsfr"] == "apic":
+        fields.append("Bij")
+    _one_step(ic, cfg, 10, fields)
+
+
+@pytest.mark.parametrize("basis", ["gimpm", "smpm"])
+def test_one_step_3d_other_bases(basis):
+    """3D GIMP and linear (smpm_3d, smpm.jl:46-66) bases."""
+    ic, cfg = slump(3, basis=basis)
+    _one_step(ic, cfg, 10, ["x", "v", "F", "J", "b", "tau", "epII"])
+
+
+def test_3d_tpic_usl_20_steps():
+    """3D TPIC without the mUSL re-projection (tpic_3d_p2n, p2n.jl:164-191)
+    over 20 steps."""
+    ic, cfg = slump(3, C_pf=1.0, cmpr=C1, trsfr="tpic", musl=False)
+    x0 = ic.mpts.x.copy(order="F")
+    ref, dev = clone(ic), clone(ic)
+    Oracle(ref.mesh, ref.mpts, ref.cmpr, cfg.instr).run(20, 100.0, plast=True)
+    r, _ = _device_run(dev, cfg, 20, 100.0)
+    assert r.steps == 20
+    err = compare(dev.mpts, ref.mpts, ["v", "F", "tau", "gradv"])
+    du = relerr(dev.mpts.x - x0, ref.mpts.x - x0)
+    print("3D TPIC-USL 20 steps", err, du)
+    assert du <= RUN_TOL and max(err.values()) <= RUN_TOL
 
 
 def test_one_step_3d_gimp_infinitesimal_j2():

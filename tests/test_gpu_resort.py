@@ -70,12 +70,21 @@ def test_resort_changes_only_rounding(case):
         assert max(compare(b.mpts, ref.mpts, fields).values()) <= 1e-9
 
 
-def test_resort_rejects_negative_and_skips_ensemble_members():
-    ic, cfg = slump(2)
-    dev = hip.Device(ic.mesh, ic.mpts, ic.cmpr, cfg.instr)
-    with pytest.raises(hip.EpError, match="EP_E_ARG"):
-        dev.set_resort(-1)
+def test_adaptive_resort_and_ensemble_members():
+    """every < 0: the overflow count is read every -every steps (here every
+    step); the result is the never-re-sorted one up to rounding.  Members of
+    an ensemble are not re-sorted."""
+    ic, cfg = slump(2, C_pf=1.0)
+    a = _run(ic, cfg, (5, 5), 0)
+    st = clone(ic)
+    dev = hip.Device(st.mesh, st.mpts, st.cmpr, cfg.instr)
+    dev.set_resort(-1)
+    assert dev.step(10, 100.0).steps == 10
+    dev.download()
+    assert dev.stats().resorts >= 0
     dev.close()
+    for f, e in compare(st.mpts, a.mpts, ["x", "v", "F", "tau", "b"]).items():
+        assert e <= 1e-11, (f, e)
     import elastoplasm as ep
     lst, _ = ep.ic_ensemble(2, 7, scales=(1,), power=1.0, precision="fp64")
     devs = []

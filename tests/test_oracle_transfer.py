@@ -156,3 +156,99 @@ def test_get_dt_rule():
     assert o.pt.vmax[0] == 0.0 and o.pt.vmax[1] == 0.0
     o.t = 99.99
     assert o.get_dt(100.0) == pytest.approx(100.0 - 99.99)
+
+
+def _solve_nodes(dim, eta, nodes):
+    """Run or_solve on the slump mesh with every node massless except
+    `nodes`: {node: (m, p[dim], f[dim], bc[dim])}; returns (a, v, oobf)."""
+    ic, cfg, o = _oracle(dim, damping=eta)
+    me = ic.mesh
+    me.m[:] = 0.0
+    me.p[:] = 7.0       # junk on massless nodes must not leak into a, v
+    me.oobf[:] = -5.0
+    o._bcs[:] = 0
+    for no, (m, p, f, bc) in nodes.items():
+        me.m[no] = m
+        me.p[:, no] = p
+        me.oobf[:, no] = f
+        o._bcs[:, no] = bc
+    load().or_solve(C.byref(o.m), 0.01, C.byref(o.ins))
+    return me.a.copy(), me.v.copy(), me.oobf.copy()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_grid_solve_undamped_closed_form(dim):
+    """η = 0 (solve.jl:14-40): a = f/m and v = (p + dt f)/m at every free
+    component; Dirichlet components and massless nodes give a = v = 0."""
+    rng = np.random.default_rng(21)
+    nodes = {}
+    for no in (3, 40, 111):
+        nodes[no] = (rng.uniform(0.5, 2.0), rng.normal(size=dim), rng.normal(size=dim) * 10, np.zeros(dim, np.uint8))
+    nodes[77] = (1.5, np.full(dim, 0.4), np.full(dim, 3.0), np.eye(dim, dtype=np.uint8)[0])   # Dirichlet in x
+    a, v, f = _solve_nodes(dim, 0.0, nodes)
+    for no, (m, p, ff, bc) in nodes.items():
+        free = bc == 0
+        np.testing.assert_allclose(a[free, no], ff[free] / m, rtol=1e-15)
+        np.testing.assert_allclose(v[free, no], (p[free] + 0.01 * ff[free]) / m, rtol=1e-14)
+        assert (a[~free, no] == 0).all() and (v[~free, no] == 0).all()
+        np.testing.assert_array_equal(f[:, no], ff)       # undamped: oobf untouched
+    massless = np.setdiff1d(np.arange(a.shape[1]), list(nodes))
+    assert (a[:, massless] == 0).all() and (v[:, massless] == 0).all()
+
+
+def test_grid_solve_gate_and_sequential_norm():
+    """Damping D = η‖oobf‖ sign(p/m) acts only where |p/m| >= 1e-3, opposes the
+    nodal velocity, and axis 2 sees the norm after axis 1 was damped
+    (solve.jl:20-31).  Hand values for η = 0.1, m = 2, f = (3, 4):
+      p = (2, 2):   f1 = 3 - 0.1*5 = 2.5; f2 = 4 - 0.1*sqrt(2.5^2 + 4^2) = 3.528301...
+      p = (-2, 2):  f1 = 3 + 0.5 = 3.5;   f2 = 4 - 0.1*sqrt(3.5^2 + 4^2)
+      p = (1e-3, 2): axis 1 below the gate (|p/m| = 5e-4), untouched;
+                     f2 = 4 - 0.1*5 = 3.5
+    A norm taken once before damping would give f2 = 3.5 in the first case."""
+    z = np.zeros(2, np.uint8)
+    f0 = np.array([3.0, 4.0])
+    nodes = {5: (2.0, np.array([2.0, 2.0]), f0, z), 9: (2.0, np.array([-2.0, 2.0]), f0, z),
+             13: (2.0, np.array([1e-3, 2.0]), f0, z)}
+    a, v, f = _solve_nodes(2, 0.1, nodes)
+    assert f[0, 5] == pytest.approx(2.5, rel=1e-15)
+    assert f[1, 5] == pytest.approx(4.0 - 0.1 * 4.716990566028302, rel=1e-15)
+    assert abs(f[1, 5] - 3.5) > 1e-3                       # not the norm-before-damping form
+    assert f[0, 9] == pytest.approx(3.5, rel=1e-15)         # damping opposes v_x < 0
+    assert f[1, 9] == pytest.approx(4.0 - 0.1 * 5.315072906367325, rel=1e-15)
+    assert f[0, 13] == 3.0                                  # gate: untouched
+    assert f[1, 13] == pytest.approx(3.5, rel=1e-15)
+    for no in nodes:
+        np.testing.assert_allclose(a[:, no], f[:, no] / 2.0, rtol=1e-15)
+        np.testing.assert_allclose(v[:, no], (nodes[no][1] + 0.01 * f[:, no]) / 2.0, rtol=1e-14)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("C_pf", [0.0, 0.37, 1.0])
+def test_picflip_blend_on_affine_grid_fields(dim, C_pf):
+    """Grid fields v_n = A x_n + c and a_n = B x_n + d (n2p.jl:14-40).  Linear
+    completeness of the basis gives Σ N v_n = A x_p + c, so C_pf = 0 (PIC)
+    sets v_p = A x_p + c, C_pf = 1 (FLIP) v_p + dt (B x_p + d), and the blend
+    is their convex combination; x_p += dt (A x_p + c) for every C_pf; vmax
+    is max |v_p| per axis."""
+    ic, cfg, o = _oracle(dim, C_pf=C_pf, musl=False)
+    mp, me = ic.mpts, ic.mesh
+    L = load()
+    assert L.or_shpfun(C.byref(o.pt), C.byref(o.m), C.byref(o.ins)) == 0
+    rng = np.random.default_rng(22)
+    A, B = rng.normal(size=(dim, dim)), rng.normal(size=(dim, dim))
+    c, d = rng.normal(size=dim), rng.normal(size=dim)
+    me.v[:] = A @ me.x + c[:, None]
+    me.a[:] = B @ me.x + d[:, None]
+    mp.s.v[:] = rng.normal(size=mp.s.v.shape)
+    v0, x0 = mp.s.v.copy(), mp.x.copy()
+    for i in range(dim):
+        o.pt.vmax[i] = 0.0
+    dt = 0.01
+    L.or_n2p(C.byref(o.pt), C.byref(o.m), dt, C.byref(o.ins))
+    pic = A @ x0 + c[:, None]
+    flip = v0 + dt * (B @ x0 + d[:, None])
+    scale = np.abs(pic).max() + np.abs(flip).max()
+    np.testing.assert_allclose(mp.s.v, C_pf * flip + (1 - C_pf) * pic, rtol=0, atol=1e-12 * scale)
+    np.testing.assert_allclose(mp.x, x0 + dt * pic, rtol=0, atol=1e-12 * np.abs(x0).max())
+    for i in range(dim):
+        assert o.pt.vmax[i] == np.abs(mp.s.v[i]).max()
