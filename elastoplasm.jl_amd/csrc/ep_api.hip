@@ -108,6 +108,8 @@ struct ep_ctx {
     int resort_every = kResortAdaptive;
     long since_sort = 0;
     int spill_seen = 0;                // ctrl->spill at the last check
+    double spill_floor = 0.0;          // overflow rate (per block and step) just after the last sort
+    bool sorted_fresh = true;          // the next check measures that floor
     long resorts = 0;
     bool in_ensemble = false;          // an ensemble holds this context's field pointers
     int *perm_d = nullptr;             // device copy of perm, permuted with the fields
@@ -361,14 +363,7 @@ static long brick_key(const ep_ctx *c, const double *x, long p) {
         if (q > g.nel[i] - 1) q = g.nel[i] - 1;
         e[i] = long(q);
     }
-    if (D == 2) {
-        const long nbz = (g.nel[1] + b[1] - 1) / b[1];
-        const long brick = e[1] / b[1] + nbz * (e[0] / b[0]);
-        return brick * (b[0] * b[1]) + (e[1] % b[1]) + b[1] * (e[0] % b[0]);
-    }
-    const long nbz = (g.nel[2] + b[2] - 1) / b[2], nbx = (g.nel[0] + b[0] - 1) / b[0];
-    const long brick = e[2] / b[2] + nbz * (e[0] / b[0] + nbx * (e[1] / b[1]));
-    return brick * (b[0] * b[1] * b[2]) + (e[2] % b[2]) + b[2] * ((e[0] % b[0]) + b[0] * (e[1] % b[1]));
+    return cell_key(D, e, b[0], b[1], b[2], g.nel);
 }
 
 // Particle fields in device order, one allocation per field, stored as T.
@@ -482,6 +477,7 @@ static int upload_fields(ep_ctx *c, const ep_particles_view *v, PState<T> &P) {
     c->sort_tmp = nullptr;
     c->sort_tmp_bytes = 0;
     c->since_sort = 0;
+    c->sorted_fresh = true;
     c->perm_dirty = false;
 #undef PUT
 #undef PUTD
@@ -941,13 +937,21 @@ extern "C" int ep_step(ep_ctx *c, int32_t nsteps, const ep_step_params *par, ep_
                 HIPCHK(hipMemcpyAsync(&spill, &c->ctrl->spill, sizeof(int), hipMemcpyDeviceToHost, c->stream));
                 HIPCHK(hipStreamSynchronize(c->stream));
                 const long nblk = (c->nslot + c->tb - 1) / c->tb;
-                go = double(spill - c->spill_seen) > kSpillFrac * double(nblk) * double(c->since_sort);
+                const double rate = double(spill - c->spill_seen) / (double(nblk) * double(c->since_sort));
                 c->spill_seen = spill;
+                if (c->sorted_fresh) {
+                    // the overflow a fresh sort leaves is the floor a re-sort cannot lower
+                    c->spill_floor = rate;
+                    c->sorted_fresh = false;
+                } else {
+                    go = rate > kSpillFrac && rate > 2.0 * c->spill_floor;
+                }
             }
             if (go) {
                 const int r = resort(c);
                 if (r != EP_OK) return r;
                 c->resorts++;
+                c->sorted_fresh = true;
             }
             c->since_sort = 0;
         }
@@ -1491,6 +1495,30 @@ int ens_restore(ep_ctx *c, HostState &h) {
 }
 
 }  // namespace
+
+// Launch shape of one context: threads per particle / node block and cells
+// per particle-sort brick; the particles are re-sorted into the new brick
+// order at once.
+extern "C" int ep_set_layout(ep_ctx *c, int32_t block, const int32_t *brick) {
+    if (!c || !brick) return fail(EP_E_ARG, "ep_set_layout: null argument");
+    if (c->in_ensemble) return fail(EP_E_STATE, "ep_set_layout: the context belongs to an ensemble");
+    if (block != 64 && block != 128 && block != 256 && block != 512)
+        return fail(EP_E_ARG, "ep_set_layout: block must be 64, 128, 256 or 512");
+    for (int i = 0; i < c->dim; i++)
+        if (brick[i] < 1 || brick[i] > 64) return fail(EP_E_ARG, "ep_set_layout: brick sides must be 1..64");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int r = ctx_set_block(c, block);
+    if (r != EP_OK) return r;
+    for (int i = 0; i < 3; i++) c->brick[i] = i < c->dim ? brick[i] : 1;
+    if (c->part_ready) {
+        const int q = resort(c);
+        if (q != EP_OK) return q;
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->since_sort = 0;
+        c->sorted_fresh = true;
+    }
+    return EP_OK;
+}
 
 extern "C" int ep_ensemble_create(ep_ctx *const *ctxs, int32_t n, ep_ensemble **out) {
     if (!ctxs || !out || n <= 0) return fail(EP_E_ARG, "ep_ensemble_create: bad argument");

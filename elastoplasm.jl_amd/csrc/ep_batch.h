@@ -56,35 +56,35 @@ __global__ void kb_augv(const Inst<T> *I, BlockMap m) {
 }
 
 template <int D, int B, typename T>
-__global__ void __launch_bounds__(256) kb_p2g(const Inst<T> *I, BlockMap m) {
+__global__ void __launch_bounds__(EP_PBLK_MAX) kb_p2g(const Inst<T> *I, BlockMap m) {
     const int i = m.inst[blockIdx.x];
     const Inst<T> &in = I[i];
     b_p2g<D, B, T>(in.c, in.g, in.P, in.G, in.M, long(blockIdx.x) - m.first[i]);
 }
 
 template <int D, int B, typename T>
-__global__ void __launch_bounds__(256) kb_g2p(const Inst<T> *I, BlockMap m) {
+__global__ void __launch_bounds__(EP_PBLK_MAX) kb_g2p(const Inst<T> *I, BlockMap m) {
     const int i = m.inst[blockIdx.x];
     const Inst<T> &in = I[i];
     b_g2p<D, B, T>(in.c, in.g, in.P, in.G, in.M, long(blockIdx.x) - m.first[i]);
 }
 
 template <int D, int B, typename T>
-__global__ void __launch_bounds__(256) kb_bij(const Inst<T> *I, BlockMap m) {
+__global__ void __launch_bounds__(EP_PBLK_MAX) kb_bij(const Inst<T> *I, BlockMap m) {
     const int i = m.inst[blockIdx.x];
     const Inst<T> &in = I[i];
     b_bij<D, B, T>(in.c, in.g, in.P, in.G, long(blockIdx.x) - m.first[i]);
 }
 
 template <int D, int B, typename T, bool LOCK>
-__global__ void __launch_bounds__(256) kb_deform(const Inst<T> *I, BlockMap m) {
+__global__ void __launch_bounds__(EP_PBLK_MAX) kb_deform(const Inst<T> *I, BlockMap m) {
     const int i = m.inst[blockIdx.x];
     const Inst<T> &in = I[i];
     b_deform<D, B, T, LOCK, false>(in.c, in.g, in.P, in.G, in.M, long(blockIdx.x) - m.first[i]);
 }
 
 template <int D, int B, typename T>
-__global__ void __launch_bounds__(256) kb_fbar(const Inst<T> *I, BlockMap m) {
+__global__ void __launch_bounds__(EP_PBLK_MAX) kb_fbar(const Inst<T> *I, BlockMap m) {
     const int i = m.inst[blockIdx.x];
     const Inst<T> &in = I[i];
     b_fbar<D, B, T, false>(in.c, in.g, in.P, in.G, in.M, long(blockIdx.x) - m.first[i]);

@@ -16,6 +16,8 @@
 namespace ep {
 
 constexpr int MAXD = 3;
+// largest block of the particle kernels (threads); ep_set_layout chooses 64..512
+#define EP_PBLK_MAX 512
 
 struct Geom {
     int nno[3];
@@ -256,6 +258,9 @@ __device__ __forceinline__ T mass_of(const PState<T> &P, const Mat &M, long p, T
 // the kernels fall back to global loads / global atomics.
 // -------------------------------------------------------------------------
 constexpr int ACC_LDS = 4096;   // doubles of LDS per block (32 KiB)
+// P2G's box (7 fields in 3D): 39.9 KiB, so that 4 blocks still share a CU's
+// 160 KiB as with ACC_LDS (occupancy is LDS-bound either way)
+constexpr int ACC_LDS_P2G = 5104;
 // Box layout.  The LDS serves a 64-bit access as four quarter-waves of 16
 // lanes, each through 32 four-byte banks, i.e. 16 bank pairs per double; the
 // measured bank-conflict counts of the walks match that model exactly.
@@ -272,11 +277,14 @@ struct BlockBox {
     int lo[D], ext[D];
     int nloc;
     int nz4, nx4, ey;          // box extent in groups of 4 along z and x; y extent (1 in 2D)
+    int cap;                   // doubles of LDS behind buf
     bool any;                  // the block has a live particle
     EP_LDS double *buf;
 
-    __device__ __forceinline__ void begin(EP_LDS double *smem, int *sint, bool live, const int *base) {
+    __device__ __forceinline__ void begin(EP_LDS double *smem, int *sint, bool live, const int *base,
+                                          int capacity = ACC_LDS) {
         buf = smem;
+        cap = capacity;
         if (threadIdx.x < D) { sint[threadIdx.x] = 0x7fffffff; sint[D + threadIdx.x] = -0x7fffffff; }
         __syncthreads();
         if (live) {
@@ -290,12 +298,12 @@ struct BlockBox {
         for (int i = 0; i < D; i++) {
             lo[i] = sint[i];
             const long e = long(sint[D + i]) - long(sint[i]) + 4;
-            ok = ok && e >= 4 && e <= ACC_LDS;
+            ok = ok && e >= 4 && e <= cap;
             ext[i] = ok ? int(e) : 4;
             n *= ext[i];
         }
         any = sint[0] != 0x7fffffff;
-        nloc = (ok && n <= ACC_LDS) ? int(n) : 0;
+        nloc = (ok && n <= cap) ? int(n) : 0;
         nz4 = (ext[D - 1] + 3) >> 2;
         nx4 = (ext[0] + 3) >> 2;
         ey = D == 3 ? ext[1] : 1;
@@ -303,7 +311,7 @@ struct BlockBox {
     }
     // doubles one field group of the layout spans (NV fields: NV times this)
     __device__ __forceinline__ int span() const { return 16 * nz4 * nx4 * ey; }
-    __device__ __forceinline__ bool fits(int nv) const { return nloc > 0 && long(span()) * nv <= ACC_LDS; }
+    __device__ __forceinline__ bool fits(int nv) const { return nloc > 0 && long(span()) * nv <= cap; }
     // address of field 0 of local node (z, x, y) in an NV-field box
     template <int NV>
     __device__ __forceinline__ int addr(int z, int x, int y) const {
@@ -503,8 +511,7 @@ __device__ __forceinline__ void for_stencil_box(const BlockBox<D> &b, const W<D,
         }
     } else {
         const int sy = 16 * NV * b.nz4 * b.nx4;
-#pragma unroll
-        for (int k0 = 0; k0 < 4; k0++) {
+        auto plane = [&](int k0) {
             const int k = ROT ? ((k0 + ry) & 3) : k0;
             const T wy = ROT ? sel4(w.w[1], k) : w.w[1][k0];
             const T dwy = ROT ? sel4(w.dw[1], k) : w.dw[1][k0];
@@ -526,6 +533,15 @@ __device__ __forceinline__ void for_stencil_box(const BlockBox<D> &b, const W<D,
                     fn(0L, row + az[j0], N, dN, ix);
                 }
             }
+        };
+        if constexpr (ROT) {
+            // scatters: one y plane (16 nodes) per iteration keeps the
+            // register count of the 7-field P2G scatter down
+#pragma unroll 1
+            for (int k0 = 0; k0 < 4; k0++) plane(k0);
+        } else {
+#pragma unroll
+            for (int k0 = 0; k0 < 4; k0++) plane(k0);
         }
     }
 }
@@ -710,7 +726,7 @@ template <int D, int B, typename T>
 __device__ __forceinline__ void b_p2g(const Ctrl *c, const Geom &g, const PState<T> &P, const GState &G, const Mat &M, long blk) {
     constexpr int NS = D == 2 ? 3 : 6;
     constexpr int NV = 1 + 2 * D;
-    __shared__ double smem[ACC_LDS];
+    __shared__ double smem[ACC_LDS_P2G];
     __shared__ int sint[8];
     if (!c->active) return;
     const long p = blk * long(blockDim.x) + threadIdx.x;
@@ -724,11 +740,9 @@ __device__ __forceinline__ void b_p2g(const Ctrl *c, const Geom &g, const PState
     W<D, T> w;
     if (!weights<D, B, T>(g, xp, lp, w) && live) atomicOr(const_cast<unsigned *>(&c->err), ERR_DOMAIN);
     BlockBox<D> box;
-    box.begin((EP_LDS double *)smem, sint, live, w.base);
-    if (!live) w.retire(box.lo);
+    box.begin((EP_LDS double *)smem, sint, live, w.base, ACC_LDS_P2G);
     const bool lds = box.fits(NV);
-    if (lds) box.zero(NV);
-    else if (threadIdx.x == 0 && box.any) atomicAdd(const_cast<int *>(&c->spill), 1);
+    if (!lds && threadIdx.x == 0 && box.any) atomicAdd(const_cast<int *>(&c->spill), 1);
     double *dst[NV];
     dst[0] = G.m;
 #pragma unroll
@@ -788,14 +802,28 @@ __device__ __forceinline__ void b_p2g(const Ctrl *c, const Geom &g, const PState
         }
     });
     };
-    if (lds) scatter(LdsSink<D, NV>{&box});
-    else scatter(GlobSink{dst});
-    if (lds) box.template flush<NV>(g, dst);
+    // When the block's particles are spread too far for one box, one pass
+    // per wave, each with the box of that wave's particles; a wave whose box
+    // still does not fit scatters with global atomics.
+    const int wave = threadIdx.x >> 6;
+    const int npass = lds ? 1 : (blockDim.x + 63) >> 6;
+    for (int pass = 0; pass < npass; pass++) {
+        const bool mine = lds || wave == pass;
+        if (!lds) box.begin((EP_LDS double *)smem, sint, live && mine, w.base, ACC_LDS_P2G);
+        const bool sl = box.fits(NV);
+        if (sl) box.zero(NV);
+        if (mine) {
+            if (!live) w.retire(box.lo);
+            if (sl) scatter(LdsSink<D, NV>{&box});
+            else scatter(GlobSink{dst});
+        }
+        if (sl) box.template flush<NV>(g, dst);
+    }
 }
 
 // kernel of one context: block b of the launch is block b of the particles / nodes
 template <int D, int B, typename T>
-__global__ void __launch_bounds__(256) k_p2g(const Ctrl *c, Geom g, PState<T> P, GState G, Mat M) { b_p2g<D, B, T>(c, g, P, G, M, blockIdx.x); }
+__global__ void __launch_bounds__(EP_PBLK_MAX) k_p2g(const Ctrl *c, Geom g, PState<T> P, GState G, Mat M) { b_p2g<D, B, T>(c, g, P, G, M, blockIdx.x); }
 
 // euler (solve.jl:14-40); a and v of massless nodes are the zeros that
 // solve() writes first (solve.jl:58-59).
@@ -921,7 +949,7 @@ __device__ __forceinline__ void b_g2p(Ctrl *c, const Geom &g, const PState<T> &P
 
 // kernel of one context: block b of the launch is block b of the particles / nodes
 template <int D, int B, typename T>
-__global__ void __launch_bounds__(256) k_g2p(Ctrl *c, Geom g, PState<T> P, GState G, Mat M) { b_g2p<D, B, T>(c, g, P, G, M, blockIdx.x); }
+__global__ void __launch_bounds__(EP_PBLK_MAX) k_g2p(Ctrl *c, Geom g, PState<T> P, GState G, Mat M) { b_g2p<D, B, T>(c, g, P, G, M, blockIdx.x); }
 
 // Bij (Bij.jl:14-26, n2p.jl:59-62): APIC's Bᵢⱼ = Σ N v_n Δnpᵀ from the
 // Euler grid velocity at the position before the PIC/FLIP update (the other
@@ -979,7 +1007,32 @@ __device__ __forceinline__ void b_bij(const Ctrl *c, const Geom &g, const PState
 }
 
 template <int D, int B, typename T>
-__global__ void __launch_bounds__(256) k_bij(const Ctrl *c, Geom g, PState<T> P, GState G) { b_bij<D, B, T>(c, g, P, G, blockIdx.x); }
+__global__ void __launch_bounds__(EP_PBLK_MAX) k_bij(const Ctrl *c, Geom g, PState<T> P, GState G) { b_bij<D, B, T>(c, g, P, G, blockIdx.x); }
+
+// Sort key of a cell (e0, e1[, e2]) in brick-major order: bricks of b0 x b1
+// (x b2) cells; inside a brick cells run z fastest, as the grid.  Bricks
+// follow a snake: along z within a column of bricks, the direction reversing
+// from one column to the next, the columns along x reversing from one y row
+// to the next, so consecutive bricks always share a face and a block of
+// particles that straddles two bricks still touches a compact node box.
+__host__ __device__ inline long cell_key(int D, const long *e, long b0, long b1, long b2, const int *nel) {
+    if (D == 2) {
+        const long nbz = (nel[1] + b1 - 1) / b1;
+        const long bx = e[0] / b0;
+        long bz = e[1] / b1;
+        if (bx & 1) bz = nbz - 1 - bz;
+        const long brick = bz + nbz * bx;
+        return brick * (b0 * b1) + (e[1] % b1) + b1 * (e[0] % b0);
+    }
+    const long nbz = (nel[2] + b2 - 1) / b2, nbx = (nel[0] + b0 - 1) / b0;
+    const long by = e[1] / b1;
+    long bx = e[0] / b0, bz = e[2] / b2;
+    if (by & 1) bx = nbx - 1 - bx;
+    const long col = bx + nbx * by;
+    if (col & 1) bz = nbz - 1 - bz;
+    const long brick = bz + nbz * col;
+    return brick * (b0 * b1 * b2) + (e[2] % b2) + b2 * ((e[0] % b0) + b0 * (e[1] % b1));
+}
 
 // Re-sort.  Particles drift out of the brick-major cell order the upload
 // gives them, and each block's node box grows with the spread of its
@@ -1002,16 +1055,7 @@ __global__ void __launch_bounds__(256) k_sort_keys(const Ctrl *c, Geom g, PState
             if (q > double(g.nel[i] - 1)) q = double(g.nel[i] - 1);
             e[i] = long(q);
         }
-        long kk;
-        if (D == 2) {
-            const long nbz = (g.nel[1] + b1 - 1) / b1;
-            const long brick = e[1] / b1 + nbz * (e[0] / b0);
-            kk = brick * (long(b0) * b1) + (e[1] % b1) + b1 * (e[0] % b0);
-        } else {
-            const long nbz = (g.nel[2] + b2 - 1) / b2, nbx = (g.nel[0] + b0 - 1) / b0;
-            const long brick = e[2] / b2 + nbz * (e[0] / b0 + nbx * (e[1] / b1));
-            kk = brick * (long(b0) * b1 * b2) + (e[2] % b2) + b2 * ((e[0] % b0) + b0 * (e[1] % b1));
-        }
+        const long kk = cell_key(D, e, b0, b1, b2, g.nel);
         k = unsigned(kk);
     }
     key[p] = k;
@@ -1489,7 +1533,7 @@ __device__ __forceinline__ void b_deform(Ctrl *c, const Geom &g, const PState<T>
 
 // kernel of one context: block b of the launch is block b of the particles / nodes
 template <int D, int B, typename T, bool LOCK, bool SURR = false>
-__global__ void __launch_bounds__(256) k_deform(Ctrl *c, Geom g, PState<T> P, GState G, Mat M) { b_deform<D, B, T, LOCK, SURR>(c, g, P, G, M, blockIdx.x); }
+__global__ void __launch_bounds__(EP_PBLK_MAX) k_deform(Ctrl *c, Geom g, PState<T> P, GState G, Mat M) { b_deform<D, B, T, LOCK, SURR>(c, g, P, G, M, blockIdx.x); }
 
 // F-bar (volumetric.jl:12-35) then elast/plast.  The box stages the grid
 // velocity and ΔJ_n/m_n exactly as ΔJs writes it (0 where m_n = 0); ΔF,
@@ -1583,6 +1627,6 @@ __device__ __forceinline__ void b_fbar(Ctrl *c, const Geom &g, const PState<T> &
 
 // kernel of one context: block b of the launch is block b of the particles / nodes
 template <int D, int B, typename T, bool SURR = false>
-__global__ void __launch_bounds__(256) k_fbar(Ctrl *c, Geom g, PState<T> P, GState G, Mat M) { b_fbar<D, B, T, SURR>(c, g, P, G, M, blockIdx.x); }
+__global__ void __launch_bounds__(EP_PBLK_MAX) k_fbar(Ctrl *c, Geom g, PState<T> P, GState G, Mat M) { b_fbar<D, B, T, SURR>(c, g, P, G, M, blockIdx.x); }
 
 }  // namespace ep

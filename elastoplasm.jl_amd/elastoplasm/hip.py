@@ -73,7 +73,7 @@ class EpStats(C.Structure):
 
 
 EXPORTS = ["ep_version", "ep_last_error", "ep_create", "ep_destroy", "ep_upload_grid", "ep_upload_particles",
-           "ep_download_particles", "ep_download_grid", "ep_set_time", "ep_set_resort", "ep_step", "ep_get_dts", "ep_profile",
+           "ep_download_particles", "ep_download_grid", "ep_set_time", "ep_set_resort", "ep_set_layout", "ep_step", "ep_get_dts", "ep_profile",
            "ep_kernel_times", "ep_get_stats", "ep_surrogate_create", "ep_surrogate_destroy",
            "ep_surrogate_get_params", "ep_surrogate_set_params", "ep_surrogate_sample", "ep_surrogate_get_samples",
            "ep_surrogate_train", "ep_surrogate_grad", "ep_surrogate_predict", "ep_attach_surrogate",
@@ -110,6 +110,7 @@ def load() -> C.CDLL:
     L.ep_download_grid.argtypes = [C.c_void_p, C.POINTER(EpGridFields)]
     L.ep_set_time.argtypes = [C.c_void_p, C.c_double]
     L.ep_set_resort.argtypes = [C.c_void_p, C.c_int32]
+    L.ep_set_layout.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]
     L.ep_step.argtypes = [C.c_void_p, C.c_int32, C.POINTER(EpStepParams), C.POINTER(EpStepResult)]
     L.ep_get_dts.argtypes = [C.c_void_p, _dp, C.c_int64, C.POINTER(C.c_int64)]
     L.ep_profile.argtypes = [C.c_void_p, C.c_int32]
@@ -225,6 +226,11 @@ class Device:
         steps; every < 0 (the library's default, -20): adaptively, checking
         the P2G box overflow count every -every steps; 0: never."""
         _check(self.L.ep_set_resort(self.h, int(every)))
+
+    def set_layout(self, block: int, brick):
+        """Threads per block and cells per sort brick (ep_set_layout)."""
+        b = (C.c_int32 * 3)(*[int(v) for v in brick] + [1] * (3 - len(brick)))
+        _check(self.L.ep_set_layout(self.h, int(block), b))
 
     def set_time(self, t: float):
         _check(self.L.ep_set_time(self.h, float(t)))

@@ -174,6 +174,11 @@ int ep_download_grid(ep_ctx *ctx, ep_grid_fields *g);
  * order); results change only in the summation order of the scatters.
  * Contexts in an ensemble are not re-sorted. */
 int ep_set_resort(ep_ctx *ctx, int32_t every);
+/* Launch shape (no counterpart in the reference): threads per particle /
+ * node block (64, 128, 256 or 512) and cells per particle-sort brick
+ * (brick[dim]); the particles are re-sorted into the new order at once.
+ * Default 256 threads, bricks of 4x2x4 (3D) / 8x8 (2D) cells. */
+int ep_set_layout(ep_ctx *ctx, int32_t block, const int32_t *brick);
 /* Time state: time.t[1] (and the step counter). */
 int ep_set_time(ep_ctx *ctx, double t);
 
@@ -196,7 +201,6 @@ int ep_get_stats(ep_ctx *ctx, ep_stats *st);
  * the harmonics (seeded), so the field is reproducible. */
 int ep_grf_gauss(int32_t dim, int64_t n, const double *x, int32_t nh, const double *waves, const double *ab,
                  double *g);
-
 /* ---- ensemble (no counterpart in the reference) ------------------------------
  * Many independent contexts (e.g. slumps of different sizes and materials,
  * each created and uploaded as usual) advanced together: one step of every
