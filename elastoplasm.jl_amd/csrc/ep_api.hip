@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <type_traits>
 #include <string>
@@ -1684,4 +1685,71 @@ extern "C" int ep_grf_gauss(int32_t dim, int64_t n, const double *x, int32_t nh,
         hipMemcpy(g, dg, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)
         return done(fail(EP_E_HIP, "random field evaluation failed"));
     return done(EP_OK);
+}
+
+// ---------------------------------------------------------------------------
+// ep_grf_gauss_noalloc: the random field with device-resident points and
+// values, for callers that cannot allocate.  The harmonic table (the staging
+// buffers ep_grf_gauss allocates) lives in `scratch`, a static device array
+// of GRF_NH_MAX * 5 doubles (640 KiB) that each GPU the library runs on holds
+// until the process ends.  Host code never uses the array's name as a device
+// address: its address on each device comes from hipGetSymbolAddress, taken
+// once per device with that device current.  Every call on a GPU overwrites
+// that GPU's table, so calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+constexpr int GRF_NH_MAX = 16384;
+static __device__ __align__(256) double scratch[GRF_NH_MAX * 5];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};
+
+int scratch_on_current_device(double **out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+        return fail(EP_E_HIP, "ep_grf_gauss_noalloc: no current device");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(EP_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch[dev] = static_cast<double *>(ptr);
+    }
+    *out = g_scratch[dev];
+    return EP_OK;
+}
+
+bool device_resident(const void *p) {
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+}  // namespace
+
+extern "C" int ep_grf_gauss_noalloc(int32_t dim, int64_t n, const double *x, int32_t nh, const double *waves,
+                                    const double *ab, double *g) {
+    if (!x || !waves || !ab || !g || n < 0 || nh <= 0 || dim < 1 || dim > 3)
+        return fail(EP_E_ARG, "ep_grf_gauss_noalloc: bad argument");
+    if (nh > GRF_NH_MAX)
+        return fail(EP_E_ARG, "ep_grf_gauss_noalloc: nh = " + std::to_string(nh) + " exceeds the static harmonic table (" +
+                                  std::to_string(GRF_NH_MAX) + ")");
+    if (n == 0) return EP_OK;
+    if (!device_resident(x) || !device_resident(g))
+        return fail(EP_E_ARG, "ep_grf_gauss_noalloc: x and g must be device memory");
+    double *scr = nullptr;
+    const int r = scratch_on_current_device(&scr);
+    if (r != EP_OK) return r;
+    double *dw = scr, *dab = scr + size_t(dim) * nh;
+    if (hipMemcpy(dw, waves, sizeof(double) * dim * nh, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dab, ab, sizeof(double) * 2 * nh, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(EP_E_HIP, "ep_grf_gauss_noalloc: staging the harmonics failed");
+    EP_LAUNCH(k_grf, (n + EP_BLOCK - 1) / EP_BLOCK, EP_BLOCK, 0, int(dim), long(n), x, int(nh), dw, dab, g);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(0) != hipSuccess)
+        return fail(EP_E_HIP, "ep_grf_gauss_noalloc: random field evaluation failed");
+    return EP_OK;
 }

@@ -246,6 +246,27 @@ inline hipError_t hipEventDestroy(hipEvent_t e) {
     return hipSuccess;
 }
 inline hipError_t hipGetLastError() { return hipSuccess; }
+// static device arrays are host arrays here; their "device address" is their address
+#define __align__(n) alignas(n)
+#define HIP_SYMBOL(X) X
+template <class T>
+inline hipError_t hipGetSymbolAddress(void **p, const T &sym) {
+    *p = const_cast<void *>(static_cast<const void *>(&sym));
+    return hipSuccess;
+}
+inline hipError_t hipGetDevice(int *d) {
+    *d = 0;
+    return hipSuccess;
+}
+enum hipMemoryType { hipMemoryTypeUnregistered = 0, hipMemoryTypeHost = 1, hipMemoryTypeDevice = 2,
+                     hipMemoryTypeManaged = 3 };
+struct hipPointerAttribute_t {
+    hipMemoryType type;
+};
+inline hipError_t hipPointerGetAttributes(hipPointerAttribute_t *a, const void *) {
+    a->type = hipMemoryTypeDevice;   // every buffer of this build is host memory standing in for device memory
+    return hipSuccess;
+}
 inline const char *hipGetErrorString(hipError_t) { return "cpu check build"; }
 
 // One persistent team of host threads per block size; the launching thread

@@ -78,7 +78,7 @@ EXPORTS = ["ep_version", "ep_last_error", "ep_create", "ep_destroy", "ep_upload_
            "ep_surrogate_get_params", "ep_surrogate_set_params", "ep_surrogate_sample", "ep_surrogate_get_samples",
            "ep_surrogate_train", "ep_surrogate_grad", "ep_surrogate_predict", "ep_attach_surrogate",
            "ep_ensemble_create", "ep_ensemble_destroy", "ep_ensemble_step", "ep_ensemble_tune",
-           "ep_ensemble_get_stats", "ep_grf_gauss"]
+           "ep_ensemble_get_stats", "ep_grf_gauss", "ep_grf_gauss_noalloc"]
 
 NPARAM = 34692
 _fp = C.POINTER(C.c_float)

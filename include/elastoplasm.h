@@ -201,6 +201,17 @@ int ep_get_stats(ep_ctx *ctx, ep_stats *st);
  * the harmonics (seeded), so the field is reproducible. */
 int ep_grf_gauss(int32_t dim, int64_t n, const double *x, int32_t nh, const double *waves, const double *ab,
                  double *g);
+/* ep_grf_gauss for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight loops, harnesses that forbid hipMalloc); it has no
+ * counterpart in the reference.  x [dim, n] and g [n] are DEVICE pointers
+ * (checked); waves and ab are host arrays, staged into a static device table
+ * of at most 16384 harmonics (nh above that is EP_E_ARG).  Calls no
+ * hipMalloc/hipFree.  The table is per GPU and shared by every call on that
+ * GPU, so calls that overlap on one GPU race.  Synchronous on the null
+ * stream. */
+int ep_grf_gauss_noalloc(int32_t dim, int64_t n, const double *x, int32_t nh, const double *waves,
+                         const double *ab, double *g);
+
 /* ---- ensemble (no counterpart in the reference) ------------------------------
  * Many independent contexts (e.g. slumps of different sizes and materials,
  * each created and uploaded as usual) advanced together: one step of every
