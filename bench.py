@@ -84,7 +84,8 @@ def build_workload(name: str, basis_override=None):
               nonloc=dict(status=False, ls=0.5), plast=dict(status=True, constitutive="DP"))
     if width is None:
         return ep.ic_slump(L, nel, cmpr=MAT, **kw)
-    ic, cfg = ep.ic_slump(L, nel, cmpr=MAT, geometry="column", column_width=width, **kw)
+    # lattice and column filter built on the device (ep_slump_points)
+    ic, cfg = ep.ic_slump(L, nel, cmpr=MAT, geometry="column", column_width=width, device=True, **kw)
     if len(L) == 3:
         ep.grf.set_cohesion_field(ic, **GRF)
     return ic, cfg

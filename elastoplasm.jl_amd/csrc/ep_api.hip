@@ -40,9 +40,9 @@ int fail(int code, const std::string &msg) {
     } while (0)
 
 enum KernelId { K_BEGIN, K_ZERO, K_P2G, K_EULER, K_G2P, K_BIJ, K_AUGV, K_DEFORM, K_FBAR, K_SURR, K_END, K_SORT,
-                K_COUNT };
+                K_NONLOCAL, K_COUNT };
 const char *kKernelNames[K_COUNT] = {"begin", "zero_grid", "p2g", "euler", "g2p", "bij", "augv", "deform", "fbar",
-                                     "surrogate", "end", "resort"};
+                                     "surrogate", "end", "resort", "nonlocal"};
 
 // Default re-sort policy of a context: adaptive, checking the p2g box
 // overflow count every 20 steps and re-sorting when more than 0.2 % of the
@@ -128,6 +128,10 @@ struct ep_ctx {
     long brick[3] = {4, 2, 4};
     // learned return (EP_FP32_SURROGATE): queue of plastic trial states
     std::vector<void *> q_allocs;
+    // non-local regularisation: element bins of the particles, scan scratch
+    std::vector<void *> nl_allocs;
+    void *nl_tmp = nullptr;
+    size_t nl_tmp_bytes = 0;
     ep_surrogate *surr = nullptr;
     float surr_tol = 0.f;
     // control
@@ -171,6 +175,9 @@ extern "C" int ep_create(const ep_config *cfg, ep_ctx **out) {
     if (cfg->plast < EP_PLAST_NONE || cfg->plast > EP_J2) return fail(EP_E_CONFIG, "InvalidReturnMapping");
     if (cfg->precision != EP_FP64 && cfg->precision != EP_FP32 && cfg->precision != EP_FP32_SURROGATE)
         return fail(EP_E_CONFIG, "unknown precision mode");
+    if (cfg->nonlocal && !(cfg->nonlocal_ls > 0.0)) return fail(EP_E_CONFIG, "non-local regularisation needs ls > 0");
+    if (cfg->nonlocal && cfg->precision == EP_FP32_SURROGATE)
+        return fail(EP_E_CONFIG, "the learned return runs without non-local regularisation");
     if (cfg->precision == EP_FP32_SURROGATE &&
         (cfg->dim != 3 || cfg->deform != EP_FINITE || cfg->plast != EP_DP))
         return fail(EP_E_CONFIG, "the learned return needs 3D finite deformation with Drucker-Prager");
@@ -204,6 +211,10 @@ extern "C" int ep_create(const ep_config *cfg, ep_ctx **out) {
     M.musl = cfg->musl != 0;
     M.locking = cfg->locking != 0;
     M.plast = cfg->plast;
+    M.nonlocal = cfg->nonlocal != 0 && cfg->plast != EP_PLAST_NONE;
+    M.nl.ls = cfg->nonlocal_ls;
+    for (int i = 0; i < 3; i++)
+        M.nl.nnel[i] = i < cfg->dim ? int(std::ceil(cfg->nonlocal_ls / cfg->h[i])) : 0;
     Geom &g = c->geom;
     long nno_tot = 1;
     for (int i = 0; i < 3; i++) {
@@ -238,6 +249,7 @@ extern "C" void ep_destroy(ep_ctx *c) {
     for (void *a : c->part_allocs) hipFree(a);
     for (void *a : c->q_allocs) hipFree(a);
     for (void *a : c->sort_allocs) hipFree(a);
+    for (void *a : c->nl_allocs) hipFree(a);
     hipFree(c->ctrl);
     hipFree(c->dt_log);
     hipFree(c->vmax_part);
@@ -401,6 +413,7 @@ static int upload_fields(ep_ctx *c, const ep_particles_view *v, PState<T> &P) {
     if (M.domain != EP_UNDEFORMED) for (int i = 0; i < D; i++) P.l0[i] = take();
     if (M.tpic) for (int i = 0; i < DD; i++) P.gv[i] = take();
     if (M.apic) for (int i = 0; i < DD; i++) P.Bm[i] = take();
+    if (M.nonlocal) P.epII2 = take();
     // re-sort: the device slot list and one spare buffer per element type
     auto take_i = [&]() -> int * {
         void *a = nullptr;
@@ -414,6 +427,41 @@ static int upload_fields(ep_ctx *c, const ep_particles_view *v, PState<T> &P) {
     c->perm_d = take_i();
     c->spare_i = take_i();
     if (oom) return fail(EP_E_HIP, "hipMalloc of particle fields failed");
+    if (M.nonlocal) {
+        for (void *a : c->nl_allocs) hipFree(a);
+        c->nl_allocs.clear();
+        NLocal &nl = c->mat.nl;
+        nl.ncell = 1;
+        for (int i = 0; i < D; i++) nl.ncell *= c->geom.nel[i];
+        auto ti = [&](size_t k) -> int * {
+            void *a = nullptr;
+            if (hipMalloc(&a, sizeof(int) * k) != hipSuccess) { oom = true; return nullptr; }
+            c->nl_allocs.push_back(a);
+            return static_cast<int *>(a);
+        };
+        nl.count = ti(size_t(nl.ncell) + 1);
+        nl.start = ti(size_t(nl.ncell) + 1);
+        nl.list = ti(size_t(n));
+        nl.cell = ti(size_t(n));
+        nl.orig = c->perm_d;
+#ifndef EP_CPU_CHECK
+        if (!oom) {
+            size_t bytes = 0;
+            if (hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, nl.count, nl.start, int(nl.ncell + 1), c->stream) !=
+                hipSuccess)
+                oom = true;
+            void *a = nullptr;
+            if (!oom && hipMalloc(&a, bytes ? bytes : 1) == hipSuccess) {
+                c->nl_allocs.push_back(a);
+                c->nl_tmp = a;
+                c->nl_tmp_bytes = bytes;
+            } else {
+                oom = true;
+            }
+        }
+#endif
+        if (oom) return fail(EP_E_HIP, "hipMalloc of the non-local bins failed");
+    }
     P.n = n;
     // host staging in device order.  A padding slot copies a particle of its
     // cell (position, material constants) unless the field has a value for
@@ -650,6 +698,8 @@ static int download_fields(ep_ctx *c, ep_particles_view *v, const PState<T> &P, 
         if (v->sig) std::memcpy(v->sig, tau.data(), sizeof(double) * NS * np_);
     }
     if (v->epII) GET(P.epII, (v->epII[0 + 2 * p] = val, v->epII[1 + 2 * p] = val));
+    // ϵpII[2]: the non-local value of the last step (plast.jl:49-60)
+    if (v->epII && M.nonlocal) GET(P.epII2, v->epII[1 + 2 * p] = val);
     if (v->dlam) GET(P.dlam, v->dlam[p] = val);
     if (v->gradv && M.tpic) for (int i = 0; i < DD; i++) GET(P.gv[i], v->gradv[i + DD * p] = val);
     if (v->Bij && M.apic) for (int i = 0; i < DD; i++) GET(P.Bm[i], v->Bij[i + DD * p] = val);
@@ -756,6 +806,26 @@ static void launch_step(ep_ctx *c, const PState<T> &P) {
     timed(c, K_EULER, [&] { EP_LAUNCH(k_euler<D>, gg, TB, s, ct, G, M); });
     timed(c, K_G2P, [&] { EP_LAUNCH((k_g2p<D, B, T>), gp, TB, s, ct, g, P, G, M); });
     if (M.apic) timed(c, K_BIJ, [&] { EP_LAUNCH((k_bij<D, B, T>), gp, TB, s, ct, g, P, G); });
+    if (M.nonlocal && M.plast != EP_PLAST_NONE) {
+        timed(c, K_NONLOCAL, [&] {
+            const NLocal nl = M.nl;
+            const dim3 gc((nl.ncell + 1 + 255) / 256), gs((n + 255) / 256);
+            EP_LAUNCH(k_nl_zero, gc, 256, s, ct, nl.count, nl.ncell + 1);
+            EP_LAUNCH((k_nl_count<D, T>), gs, 256, s, ct, g, P, nl);
+#ifdef EP_CPU_CHECK
+            {
+                // the checking build runs kernels synchronously on host threads
+                int acc = 0;
+                for (long e = 0; e <= nl.ncell; e++) { nl.start[e] = acc; acc += nl.count[e]; }
+            }
+#else
+            hipcub::DeviceScan::ExclusiveSum(c->nl_tmp, c->nl_tmp_bytes, nl.count, nl.start, int(nl.ncell + 1), s);
+#endif
+            EP_LAUNCH(k_nl_zero, gc, 256, s, ct, nl.count, nl.ncell + 1);
+            EP_LAUNCH((k_nl_fill<T>), gs, 256, s, ct, n, nl);
+            EP_LAUNCH((k_nonlocal<D, T>), gs, 256, s, ct, g, P, nl);
+        });
+    }
     if (M.musl) timed(c, K_AUGV, [&] { EP_LAUNCH(k_augv<D>, gg, TB, s, ct, G); });
     if (M.locking) timed(c, K_DEFORM, [&] { EP_LAUNCH((k_deform<D, B, T, true>), gp, TB, s, ct, g, P, G, M); });
     else timed(c, K_DEFORM, [&] { EP_LAUNCH((k_deform<D, B, T, false, SURR>), gp, TB, s, ct, g, P, G, M); });
@@ -862,6 +932,7 @@ static int resort_t(ep_ctx *c, PState<T> &P) {
     gather(P.phi, spare);
     for (int i = 0; i < D; i++) { gather(P.l[i], spare); gather(P.l0[i], spare); }
     for (int i = 0; i < DD; i++) { gather(P.gv[i], spare); gather(P.Bm[i], spare); }
+    gather(P.epII2, spare);
     gather(c->perm_d, c->spare_i);
     c->perm_dirty = true;
     return EP_OK;
@@ -1536,6 +1607,8 @@ extern "C" int ep_ensemble_create(ep_ctx *const *ctxs, int32_t n, ep_ensemble **
     }
     if (c0.basis != EP_BSMPM) return fail(EP_E_CONFIG, "ensembles run the cubic B-spline basis");
     if (c0.precision == EP_FP32_SURROGATE) return fail(EP_E_CONFIG, "ensembles run EP_FP64 or EP_FP32");
+    for (int k = 0; k < n; k++)
+        if (ctxs[k]->mat.nonlocal) return fail(EP_E_CONFIG, "ensembles run without non-local regularisation");
     ep_ensemble *e = new ep_ensemble();
     e->ctx.assign(ctxs, ctxs + n);
     // the ensemble's instance table holds the contexts' field pointers
@@ -1630,6 +1703,130 @@ extern "C" int ep_ensemble_get_stats(ep_ensemble *e, ep_stats *st) {
     if (!e || !st) return fail(EP_E_ARG, "null argument");
     *st = e->stats;
     return EP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Particle lattice and slump geometry on the device: mpts_populate's lattice
+// (mpts_populate.jl:26-56) filtered as get_slump does (get_slump.jl:18-95)
+// or to a rectangular column, kept in lattice order.
+// ---------------------------------------------------------------------------
+namespace {
+struct LatticeSpec {
+    int dim, geometry;       // geometry: 0 wedge (get_slump), 1 column
+    long n[3];               // points per axis (x, y, z in 3D; x, z in 2D)
+    const double *ax[3];     // per-axis coordinates
+    double par[5];           // wedge: zcut, wl, a, X, Z; column: xmax, zmax
+};
+
+// The lattice runs along the last axis fastest, then x, then y (the vec
+// order of mpts_populate.jl:34-43).
+__device__ __host__ inline void lattice_point(const LatticeSpec &L, long i, double *x) {
+    const long nz = L.n[L.dim - 1];
+    const long iz = i % nz, r = i / nz;
+    if (L.dim == 2) {
+        x[0] = L.ax[0][r];
+        x[1] = L.ax[1][iz];
+    } else {
+        x[0] = L.ax[0][r % L.n[0]];
+        x[1] = L.ax[1][r / L.n[0]];
+        x[2] = L.ax[2][iz];
+    }
+}
+
+__device__ __host__ inline bool lattice_keep(const LatticeSpec &L, const double *x) {
+#pragma clang fp contract(off)
+    const double xp = x[0], zp = x[L.dim - 1];
+    if (L.geometry == 1) return xp <= L.par[0] && zp <= L.par[1];
+    if (!(zp <= L.par[0])) return false;
+    // Δx nx + Δz nz of the last line point, nx = a, nz = -1 (get_slump.jl:30-41, 59-71)
+    const double dx = xp - L.par[3], dz = zp - L.par[4];
+    const double s = dx * L.par[2] + dz * -1.0;
+    return s > 0.0 || zp < L.par[1];
+}
+
+__global__ void k_lattice_flags(LatticeSpec L, long n, unsigned char *flag) {
+    const long i = blockIdx.x * long(blockDim.x) + threadIdx.x;
+    if (i >= n) return;
+    double x[3];
+    lattice_point(L, i, x);
+    flag[i] = lattice_keep(L, x) ? 1 : 0;
+}
+
+__global__ void k_lattice_gather(LatticeSpec L, const int *sel, long m, double *out) {
+    const long k = blockIdx.x * long(blockDim.x) + threadIdx.x;
+    if (k >= m) return;
+    double x[3];
+    lattice_point(L, sel[k], x);
+    for (int d = 0; d < L.dim; d++) out[d + L.dim * k] = x[d];
+}
+}  // namespace
+
+extern "C" int ep_slump_points(int32_t dim, const double *const *axes, const int64_t *n_axis, int32_t geometry,
+                               const double *par, int64_t cap, double *x_out, int64_t *nmp) {
+    if (!axes || !n_axis || !par || !nmp || (dim != 2 && dim != 3) || (geometry != 0 && geometry != 1) || cap < 0)
+        return fail(EP_E_ARG, "ep_slump_points: bad argument");
+    LatticeSpec L{};
+    L.dim = dim;
+    L.geometry = geometry;
+    long n = 1;
+    for (int i = 0; i < dim; i++) {
+        if (!axes[i] || n_axis[i] < 1) return fail(EP_E_ARG, "ep_slump_points: empty axis");
+        L.n[i] = n_axis[i];
+        n *= n_axis[i];
+    }
+    if (n > 0x7fffffffL) return fail(EP_E_ARG, "ep_slump_points: lattice exceeds 2^31 - 1 points");
+    for (int k = 0; k < 5; k++) L.par[k] = par[k];
+    std::vector<void *> mem;
+    auto done = [&](int r) {
+        for (void *a : mem) hipFree(a);
+        return r;
+    };
+    auto alloc = [&](size_t bytes) -> void * {
+        void *a = nullptr;
+        if (hipMalloc(&a, bytes ? bytes : 1) != hipSuccess) return nullptr;
+        mem.push_back(a);
+        return a;
+    };
+    double *dax[3] = {nullptr, nullptr, nullptr};
+    for (int i = 0; i < dim; i++) {
+        dax[i] = static_cast<double *>(alloc(sizeof(double) * L.n[i]));
+        if (!dax[i] || hipMemcpy(dax[i], axes[i], sizeof(double) * L.n[i], hipMemcpyHostToDevice) != hipSuccess)
+            return done(fail(EP_E_HIP, "ep_slump_points: axis upload failed"));
+        L.ax[i] = dax[i];
+    }
+    auto *flag = static_cast<unsigned char *>(alloc(size_t(n)));
+    auto *sel = static_cast<int *>(alloc(sizeof(int) * size_t(n)));
+    auto *cnt = static_cast<int *>(alloc(sizeof(int)));
+    if (!flag || !sel || !cnt) return done(fail(EP_E_HIP, "ep_slump_points: hipMalloc failed"));
+    EP_LAUNCH(k_lattice_flags, (n + 255) / 256, 256, 0, L, n, flag);
+    int m = 0;
+#ifdef EP_CPU_CHECK
+    for (long i = 0; i < n; i++)
+        if (flag[i]) sel[m++] = int(i);
+#else
+    {
+        size_t bytes = 0;
+        hipcub::CountingInputIterator<int> it(0);
+        if (hipcub::DeviceSelect::Flagged(nullptr, bytes, it, flag, sel, cnt, int(n)) != hipSuccess)
+            return done(fail(EP_E_HIP, "ep_slump_points: selection failed"));
+        void *tmp = alloc(bytes);
+        if (!tmp || hipcub::DeviceSelect::Flagged(tmp, bytes, it, flag, sel, cnt, int(n)) != hipSuccess ||
+            hipMemcpy(&m, cnt, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+            return done(fail(EP_E_HIP, "ep_slump_points: selection failed"));
+    }
+#endif
+    *nmp = m;
+    if (m > cap) return done(fail(EP_E_ARG, "ep_slump_points: cap " + std::to_string(cap) + " < " + std::to_string(m) +
+                                                " kept points (*nmp holds the count)"));
+    if (m == 0) return done(EP_OK);
+    if (!x_out) return done(fail(EP_E_ARG, "ep_slump_points: null output"));
+    auto *dout = static_cast<double *>(alloc(sizeof(double) * dim * size_t(m)));
+    if (!dout) return done(fail(EP_E_HIP, "ep_slump_points: hipMalloc failed"));
+    EP_LAUNCH(k_lattice_gather, (m + 255) / 256, 256, 0, L, sel, long(m), dout);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpy(x_out, dout, sizeof(double) * dim * size_t(m), hipMemcpyDeviceToHost) != hipSuccess)
+        return done(fail(EP_E_HIP, "ep_slump_points: lattice evaluation failed"));
+    return done(EP_OK);
 }
 
 // ---------------------------------------------------------------------------

@@ -46,6 +46,7 @@ struct PState {
     T *l0[3];                  // ℓ₀, kept for domain updates other than `undeformed`
     T *gv[9];                  // ∇v (TPIC state)
     T *Bm[9];                  // APIC Bᵢⱼ [i + D*j] (Bij.jl:14-26)
+    T *epII2;                  // ϵpII[2]: non-local plastic strain of the step (non-local contexts)
     long n;
 };
 
@@ -85,13 +86,24 @@ struct SurrQ {
     float *V[9];
 };
 
+// Non-local regularisation: particles binned by the element of their
+// start-of-step position (p2e) each step.
+struct NLocal {
+    int *count, *start, *list, *cell;  // per element: count / first list entry; per slot: element
+    const int *orig;                   // slot -> the caller's particle index (the reference's order)
+    long ncell;
+    int nnel[3];                       // ceil(ls / h) per axis
+    double ls;
+};
+
 struct Mat {
     double Kc, Gc, Hp, c, C_pf, damping;
     double E, nu;
     double Del[36];
     DelLU lu;
-    int finite, tpic, musl, locking, plast, apic, domain;
+    int finite, tpic, musl, locking, plast, apic, domain, nonlocal;
     SurrQ q;
+    NLocal nl;
 };
 
 // -------------------------------------------------------------------------
@@ -496,7 +508,7 @@ __device__ __forceinline__ void for_stencil_box(const BlockBox<D> &b, const W<D,
         const int x = xb + i;
         ax[j0] = 16 * NV * b.nz4 * (x >> 2) + 4 * (x & 3);
     }
-    if (D == 2) {
+    if constexpr (D == 2) {
 #pragma unroll
         for (int i0 = 0; i0 < 4; i0++) {
 #pragma unroll
@@ -1289,7 +1301,7 @@ __device__ __forceinline__ void constitutive(Ctrl *c, const PState<T> &P, const 
             }
         }
         if (plast && M.plast == 1) {
-            T e2 = P.epII[p], e1 = e2, dl;
+            T e1 = P.epII[p], e2 = M.nonlocal ? P.epII2[p] : e1, dl;
             dp_return<NS, T>(sp, P.c0[p], P.cr[p], P.phi[p], T(M.Hp), T(M.Kc), T(M.Gc), e2, e1, dl);
             P.dlam[p] = dl;
             P.epII[p] = e1;
@@ -1307,7 +1319,7 @@ __device__ __forceinline__ void constitutive(Ctrl *c, const PState<T> &P, const 
         vdvt<D, T>(V, tp, tm);
         sym_to_voigt<D, T>(tm, s);
         if (plast && M.plast == 2) {
-            T e2 = P.epII[p], e1 = e2;
+            T e1 = P.epII[p], e2 = M.nonlocal ? P.epII2[p] : e1;
             const int it = j2_return<NS, T>(s, P.c0[p], P.cr[p], T(M.Hp), M.Del, e2, e1);
             if (it != 0) {
                 P.epII[p] = e1;
@@ -1351,12 +1363,12 @@ __device__ __forceinline__ void constitutive(Ctrl *c, const PState<T> &P, const 
             s[i] = P.s[i][p] + acc;
         }
         if (c->with_plast && M.plast == 1) {
-            T e2 = P.epII[p], e1 = e2, dl;
+            T e1 = P.epII[p], e2 = M.nonlocal ? P.epII2[p] : e1, dl;
             dp_return<NS, T>(s, P.c0[p], P.cr[p], P.phi[p], T(M.Hp), T(M.Kc), T(M.Gc), e2, e1, dl);
             P.dlam[p] = dl;
             P.epII[p] = e1;
         } else if (c->with_plast && M.plast == 2) {
-            T e2 = P.epII[p], e1 = e2;
+            T e1 = P.epII[p], e2 = M.nonlocal ? P.epII2[p] : e1;
             const int it = j2_return<NS, T>(s, P.c0[p], P.cr[p], T(M.Hp), M.Del, e2, e1);
             if (it != 0) {
                 P.epII[p] = e1;
@@ -1628,5 +1640,99 @@ __device__ __forceinline__ void b_fbar(Ctrl *c, const Geom &g, const PState<T> &
 // kernel of one context: block b of the launch is block b of the particles / nodes
 template <int D, int B, typename T, bool SURR = false>
 __global__ void __launch_bounds__(EP_PBLK_MAX) k_fbar(Ctrl *c, Geom g, PState<T> P, GState G, Mat M) { b_fbar<D, B, T, SURR>(c, g, P, G, M, blockIdx.x); }
+
+
+// ---------------------------------------------------------------------------
+// Non-local regularisation (nonlocal.jl:18-47 as plast.jl:49-60 runs it, the
+// neighbourhood of e2e.jl:20-61).  Per step, after G2P: the particles are
+// binned by the element of their start-of-step position (the p2e of shpfun,
+// tplgy.jl:43-96), then every plastic particle p (Δλ of the last return != 0)
+// sums over the particles q of the elements within ceil(ls/h) on every axis,
+// with ω = (d/ls) exp(-(d/ls)²) and d from the first two coordinates of the
+// updated positions:
+//   W_p = Σ_q ω,   ϵpII[2,p] = Σ_q' ω ϵpII[1,q'] / W_p  (ϵpII[1,p] if W_p <= 1e-16)
+// where q' leaves out the plastic particles that come before p in the
+// caller's order (the reference's sequential "p->q" pass has already given
+// their pair a weight and skips it).  Every other particle gets ϵpII[2] = 0.
+// ---------------------------------------------------------------------------
+template <int D>
+__device__ __forceinline__ long nl_element(const Geom &g, const double *xp, int *e) {
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+        double q = exact_fld(xp[i] - g.x0[i], g.h[i], g.ih[i]);
+        const double hi = double(g.nel[i] - 1);
+        q = q < 0.0 ? 0.0 : (q > hi ? hi : q);
+        e[i] = int(q);
+    }
+    return D == 2 ? long(e[1]) + long(g.nel[1]) * e[0] : long(e[2]) + long(g.nel[2]) * (long(e[0]) + long(g.nel[0]) * e[1]);
+}
+
+__global__ void __launch_bounds__(256) k_nl_zero(const Ctrl *c, int *a, long n) {
+    const long i = blockIdx.x * long(blockDim.x) + threadIdx.x;
+    if (i < n && c->active && c->with_plast) a[i] = 0;
+}
+
+template <int D, typename T>
+__global__ void __launch_bounds__(256) k_nl_count(const Ctrl *c, Geom g, PState<T> P, NLocal nl) {
+    const long p = blockIdx.x * long(blockDim.x) + threadIdx.x;
+    if (p >= P.n || !c->active || !c->with_plast) return;
+    if (P.Om0[p] == T(0)) { nl.cell[p] = -1; return; }
+    double xp[D];
+#pragma unroll
+    for (int i = 0; i < D; i++) xp[i] = P.x[c->parity][i][p];
+    int e[3];
+    const long el = nl_element<D>(g, xp, e);
+    nl.cell[p] = int(el);
+    atomicAdd(&nl.count[el], 1);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_nl_fill(const Ctrl *c, long n, NLocal nl) {
+    const long p = blockIdx.x * long(blockDim.x) + threadIdx.x;
+    if (p >= n || !c->active || !c->with_plast) return;
+    const int el = nl.cell[p];
+    if (el < 0) return;
+    nl.list[nl.start[el] + atomicAdd(&nl.count[el], 1)] = int(p);
+}
+
+template <int D, typename T>
+__global__ void __launch_bounds__(256) k_nonlocal(const Ctrl *c, Geom g, PState<T> P, NLocal nl) {
+    const long p = blockIdx.x * long(blockDim.x) + threadIdx.x;
+    if (p >= P.n || !c->active || !c->with_plast) return;
+    if (P.Om0[p] == T(0) || P.dlam[p] == T(0)) { P.epII2[p] = T(0); return; }
+    const int np = c->parity ^ 1;   // G2P wrote the updated positions here
+    const double x0 = P.x[np][0][p], x1 = P.x[np][1][p];
+    double xs[D];
+#pragma unroll
+    for (int i = 0; i < D; i++) xs[i] = P.x[c->parity][i][p];
+    int e[3];
+    nl_element<D>(g, xs, e);
+    const int op = nl.orig[p];
+    const double ils = 1.0 / nl.ls;
+    double W = 0.0, S = 0.0;
+    int lo[3], hi[3];
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+        lo[i] = e[i] - nl.nnel[i] < 0 ? 0 : e[i] - nl.nnel[i];
+        hi[i] = e[i] + nl.nnel[i] > g.nel[i] - 1 ? g.nel[i] - 1 : e[i] + nl.nnel[i];
+    }
+    const int ylo = D == 3 ? lo[1] : 0, yhi = D == 3 ? hi[1] : 0;
+    for (int ey = ylo; ey <= yhi; ey++)
+        for (int ex = lo[0]; ex <= hi[0]; ex++)
+            for (int ez = lo[D - 1]; ez <= hi[D - 1]; ez++) {
+                const long el = D == 2 ? long(ez) + long(g.nel[1]) * ex
+                                       : long(ez) + long(g.nel[2]) * (long(ex) + long(g.nel[0]) * ey);
+                const int a = nl.start[el], b = nl.start[el + 1];
+                for (int k = a; k < b; k++) {
+                    const int q = nl.list[k];
+                    const double xi = x0 - P.x[np][0][q], eta = x1 - P.x[np][1][q];
+                    const double r = sqrt(xi * xi + eta * eta) * ils;
+                    const double w = r * exp(-(r * r));
+                    W += w;
+                    if (!(P.dlam[q] != T(0) && nl.orig[q] < op)) S += w * double(P.epII[q]);
+                }
+            }
+    P.epII2[p] = W > 1e-16 ? T(S / W) : P.epII[p];
+}
 
 }  // namespace ep

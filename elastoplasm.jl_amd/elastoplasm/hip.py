@@ -32,7 +32,8 @@ class EpConfig(C.Structure):
                 ("h", C.c_double * 3), ("x0", C.c_double * 3), ("xB", (C.c_double * 2) * 3),
                 ("C_pf", C.c_double), ("damping", C.c_double), ("Kc", C.c_double), ("Gc", C.c_double),
                 ("Hp", C.c_double), ("c", C.c_double), ("Del", C.c_double * 36),
-                ("surrogate_tol", C.c_double), ("domain", C.c_int32), ("reserved", C.c_int32 * 7)]
+                ("surrogate_tol", C.c_double), ("domain", C.c_int32), ("nonloc", C.c_int32),
+                ("reserved", C.c_int32 * 6), ("nonloc_ls", C.c_double)]
 
 
 class EpGridView(C.Structure):
@@ -78,7 +79,7 @@ EXPORTS = ["ep_version", "ep_last_error", "ep_create", "ep_destroy", "ep_upload_
            "ep_surrogate_get_params", "ep_surrogate_set_params", "ep_surrogate_sample", "ep_surrogate_get_samples",
            "ep_surrogate_train", "ep_surrogate_grad", "ep_surrogate_predict", "ep_attach_surrogate",
            "ep_ensemble_create", "ep_ensemble_destroy", "ep_ensemble_step", "ep_ensemble_tune",
-           "ep_ensemble_get_stats", "ep_grf_gauss", "ep_grf_gauss_noalloc"]
+           "ep_ensemble_get_stats", "ep_grf_gauss", "ep_grf_gauss_noalloc", "ep_slump_points"]
 
 NPARAM = 34692
 _fp = C.POINTER(C.c_float)
@@ -179,6 +180,9 @@ def make_config(mesh: Mesh, mpts: Point, cmpr, instr) -> EpConfig:
         cfg.Del[k] = v
     cfg.surrogate_tol = float(instr.get("surrogate_tol", 1e-3))
     cfg.domain = ops.get("domain", 0)
+    nl = instr.get("nonloc", {})
+    cfg.nonloc = 1 if nl.get("status", False) else 0   # ep_config.nonlocal
+    cfg.nonloc_ls = float(np.max(nl.get("ls", 0.5)))    # ep_config.nonlocal_ls
     return cfg
 
 

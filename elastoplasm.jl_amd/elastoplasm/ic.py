@@ -204,9 +204,49 @@ def mpts_populate(mesh, cmpr, instr, ni=2):
     return NT(ni=ni, x=x, c0=c0)
 
 
-def get_slump(mesh, cmpr, instr, ni=2, lz=12.80):
-    out = mpts_populate(mesh, cmpr, instr, ni=ni)
+def _axes(mesh, ni):
+    return [frange(mesh.xB[i, 0] + (0.5 * mesh.h[i] / ni), mesh.h[i] / ni, mesh.xB[i, 1]) for i in range(mesh.dim)]
+
+
+def _device_points(mesh, ni, geometry, par):
+    """The lattice and its filter evaluated by ep_slump_points (host ranges,
+    device lattice, filter and compaction)."""
+    import ctypes as C
+    from . import hip
+    L = hip.load()
+    dp = C.POINTER(C.c_double)
+    L.ep_slump_points.argtypes = [C.c_int32, C.POINTER(dp), C.POINTER(C.c_int64), C.c_int32, dp, C.c_int64, dp,
+                                  C.POINTER(C.c_int64)]
+    d = mesh.dim
+    axes = [np.ascontiguousarray(a, dtype=np.float64) for a in _axes(mesh, ni)]
+    ap = (dp * 3)(*[a.ctypes.data_as(dp) for a in axes] + [None] * (3 - d))
+    na = (C.c_int64 * 3)(*[len(a) for a in axes] + [1] * (3 - d))
+    pr = np.zeros(5)
+    pr[:len(par)] = par
+    cap = int(np.prod([len(a) for a in axes]))
+    x = np.zeros((d, cap), order="F")
+    n = C.c_int64()
+    hip._check(L.ep_slump_points(d, ap, na, geometry, pr.ctypes.data_as(dp), cap, x.ctypes.data_as(dp), C.byref(n)))
+    return np.asfortranarray(x[:, :n.value])
+
+
+def get_slump(mesh, cmpr, instr, ni=2, lz=12.80, device=False):
+    """get_slump.jl:18-95.  device=True evaluates the lattice and the filter
+    on the GPU (ep_slump_points); the result is the same array."""
     wl = 0.15 * lz
+    if device:
+        if instr["grf"]["status"]:
+            raise ValueError("the reference's unseeded spectral GRF is not restated; "
+                             "use cohesion_field=... with a seeded field instead")
+        xs = _axes(mesh, ni)[0]
+        a = -1.25
+        xmax = float(xs.max())      # the filter line's last point: max x of the kept lattice (its lowest z row)
+        xp = _device_points(mesh, ni, 0, [lz - (0.5 * mesh.h[-1] / ni), wl, a, xmax + 0.5 * mesh.L[0], a * xmax])
+        nmp = xp.shape[1]
+        phi = np.full(nmp, cmpr.phi0)
+        phi[xp[-1, :] <= 2 * wl] = cmpr.phir
+        return NT(xp=xp, coh0=np.full(nmp, cmpr.c0), cohr=np.full(nmp, cmpr.cr), phi=phi, ni=ni, nmp=nmp)
+    out = mpts_populate(mesh, cmpr, instr, ni=ni)
     keep = out.x[-1, :] <= lz - (0.5 * mesh.h[-1] / ni)
     xp = out.x[:, keep]
     c = out.c0[keep]
@@ -225,14 +265,19 @@ def get_slump(mesh, cmpr, instr, ni=2, lz=12.80):
     return NT(xp=xp, coh0=coh0, cohr=cohr, phi=phi, ni=ni, nmp=nmp)
 
 
-def get_column(mesh, cmpr, instr, ni=2, width=0.25, height=None):
+def get_column(mesh, cmpr, instr, ni=2, width=0.25, height=None, device=False):
     """Rectangular column x <= width*L_x, z <= height (default the domain
     height L_z, i.e. below the 3D mesh's 2h headroom): the column geometry
-    of the 3D slump workloads, an addition to the reference's wedge."""
-    x = _lattice(mesh, ni)
-    keep = x[0, :] <= width * mesh.L[0]
-    keep &= x[-1, :] <= (mesh.L[-1] if height is None else height)
-    xp = np.asfortranarray(x[:, keep])
+    of the 3D slump workloads, an addition to the reference's wedge.
+    device=True evaluates it on the GPU (ep_slump_points)."""
+    zmax = mesh.L[-1] if height is None else height
+    if device:
+        xp = _device_points(mesh, ni, 1, [width * mesh.L[0], zmax])
+    else:
+        x = _lattice(mesh, ni)
+        keep = x[0, :] <= width * mesh.L[0]
+        keep &= x[-1, :] <= zmax
+        xp = np.asfortranarray(x[:, keep])
     nmp = xp.shape[1]
     return NT(xp=xp, coh0=np.full(nmp, cmpr.c0), cohr=np.full(nmp, cmpr.cr),
               phi=np.full(nmp, cmpr.phi0), ni=ni, nmp=nmp)

@@ -82,10 +82,12 @@ def elastoplasm(ic, cfg, mode: str = "elastodynamic"):
 
 
 def ic_slump(L, nel, fid: str = "slump", cmpr=None, geometry: str = "wedge", column_width: float = 0.25,
-             **kwargs):
+             device: bool = False, **kwargs):
     """slump.jl:22-49.  ``cmpr`` (material overrides for setup_cmpr) and
     ``geometry`` ('wedge' = the reference's get_slump, 'column' = a
-    quarter-width column) are additions for the cohesive column workloads."""
+    quarter-width column) are additions for the cohesive column workloads;
+    ``device`` builds the particle lattice and its geometry filter on the GPU
+    (the same arrays)."""
     instr = kwargser("instr", kwargs, dim=len(L))
     geom = get_geom(nel, L, instr)
     mesh = setup_mesh(instr, geom)
@@ -93,9 +95,9 @@ def ic_slump(L, nel, fid: str = "slump", cmpr=None, geometry: str = "wedge", col
     base = {k: cm.pop(k) for k in ("E", "nu", "rho0") if k in cm}
     cmp = setup_cmpr(mesh, **base, **cm)
     if geometry == "wedge":
-        g = get_slump(mesh, cmp, instr)
+        g = get_slump(mesh, cmp, instr, device=device)
     elif geometry == "column":
-        g = get_column(mesh, cmp, instr, width=column_width)
+        g = get_column(mesh, cmp, instr, width=column_width, device=device)
     else:
         raise ValueError(f"unknown geometry {geometry}")
     mpts = setup_mpts(mesh, cmp, g)

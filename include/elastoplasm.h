@@ -78,7 +78,9 @@ typedef struct ep_config {
     double Del[36];       /* cmpr.Del, nstr x nstr column-major */
     double surrogate_tol; /* EP_FP32_SURROGATE: yield-residual tolerance */
     int32_t domain;       /* GIMP domain update, an ep_domain value */
-    int32_t reserved[7];
+    int32_t nonlocal;     /* instr.nonloc.status: non-local plastic strain (nonlocal.jl:18-47) */
+    int32_t reserved[6];
+    double nonlocal_ls;   /* instr.nonloc.ls */
 } ep_config;
 
 /* Pointers into mesh (types.jl:141-167). */
@@ -193,6 +195,19 @@ int ep_profile(ep_ctx *ctx, int32_t enable);
 int ep_kernel_times(ep_ctx *ctx, int32_t max_kernels, char *names, int32_t name_len,
                     double *total_ms, int64_t *launches, int32_t *count);
 int ep_get_stats(ep_ctx *ctx, ep_stats *st);
+
+/* The particle lattice of mpts_populate (mpts_populate.jl:26-56) and the
+ * slump filter of get_slump (get_slump.jl:18-95) evaluated on the device.
+ * axes[i] holds the n_axis[i] particle coordinates along axis i (the Julia
+ * ranges xB_lo + h/(2 ni) : h/ni : xB_hi); the lattice runs along the last
+ * axis fastest, then x, then y.  geometry 0 (get_slump): par = {zcut, wl, a,
+ * X, Z}, a point is kept when z <= zcut and ((x - X) a - (z - Z) > 0 or
+ * z < wl), (X, Z) being the last point of the filter line; geometry 1 (a
+ * rectangular column): par = {xmax, zmax, -, -, -}, kept when x <= xmax and
+ * z <= zmax.  The kept points go to x_out [dim, cap] in lattice order and
+ * their count to *nmp (also when it exceeds cap, which is EP_E_ARG). */
+int ep_slump_points(int32_t dim, const double *const *axes, const int64_t *n_axis, int32_t geometry,
+                    const double *par, int64_t cap, double *x_out, int64_t *nmp);
 
 /* Gaussian random field at n points by spectral randomisation, the method of
  * GRFS_gauss (src/home/program/utils/setup/mpts/GRF.jl:66-143):

@@ -908,10 +908,133 @@ void or_elast(or_point *mp, const or_instr *in) {
 }
 
 /* plast (plast.jl:49-65) with the local (nonloc off) branch */
+/* Symmetric pair weights w[p,q] of nonlocal.jl: an open-addressing map from
+ * the pair (min, max) to its value; absent pairs read 0. */
+typedef struct { int64_t *key; double *val; int64_t cap; } pairmap;
+
+static int64_t pm_slot(const pairmap *m, int64_t p, int64_t q) {
+    const int64_t a = p < q ? p : q, b = p < q ? q : p;
+    const int64_t k = a * 0x100000000LL + b;
+    uint64_t h = (uint64_t)k * 0x9E3779B97F4A7C15ULL;
+    int64_t i = (int64_t)(h >> 20) & (m->cap - 1);
+    while (m->key[i] != -1 && m->key[i] != k) i = (i + 1) & (m->cap - 1);
+    m->key[i] = k;
+    return i;
+}
+
+/* nonlocal (nonlocal.jl:18-47) as plast calls it (plast.jl:49-60), in the
+ * reference's single-threaded order: "tplgy" lists, for every element, the
+ * particles whose e2e neighbourhood (e2e.jl:20-61: within ceil(ls/h)
+ * elements on every axis) holds it; "p->q" gives every plastic particle p
+ * (Δλ of the last return != 0), for each particle q of its element's list in
+ * ascending order whose weight is still 0, w = w[q,p] = (d/ls) exp(-(d/ls)^2),
+ * d from the first two coordinates, adds w to W[p] and W[q] and records q in
+ * p's p2p column; "p<-q" sets ϵpII[2,p] = Σ (w[p,q]/W[p]) ϵpII[1,q] over
+ * that column, or ϵpII[1,p] when W[p] <= 1e-16.  ϵpII[2] of every other
+ * particle is 0.  Returns -1 if memory runs out. */
+int or_nonlocal(or_point *mp, const or_mesh *me, const or_instr *in) {
+    const int dim = me->dim;
+    const int64_t nmp = mp->nmp;
+    int64_t nel[3] = {1, 1, 1}, nnel[3] = {0, 0, 0};
+    for (int i = 0; i < dim; i++) {
+        nel[i] = me->nel[i];
+        nnel[i] = (int64_t)ceil(in->ls / me->h[i]);
+    }
+    const int64_t ntot = me->nel[dim];
+    /* element coordinates from the 1-based element id (tplgy.jl:49, 82) */
+    int64_t *ec = malloc(sizeof(int64_t) * 3 * (size_t)nmp);
+    int64_t *cnt = calloc((size_t)ntot + 1, sizeof(int64_t));
+    double *W = calloc((size_t)nmp, sizeof(double));
+    if (!ec || !cnt || !W) { free(ec); free(cnt); free(W); return -1; }
+    const int zl = dim - 1;
+    for (int64_t p = 0; p < nmp; p++) {
+        int64_t e = mp->p2e[p] - 1;
+        ec[3 * p + zl] = e % nel[zl];
+        e /= nel[zl];
+        ec[3 * p + 0] = e % nel[0];
+        if (dim == 3) ec[3 * p + 1] = e / nel[0];
+    }
+#define EL_ID(x, y, z) (dim == 2 ? (z) + nel[1] * (x) : (z) + nel[2] * ((x) + nel[0] * (y)))
+#define NBR_LOOP(p, BODY)                                                                          \
+    for (int64_t yy = (dim == 3 ? ec[3 * (p) + 1] - nnel[1] : 0); yy <= (dim == 3 ? ec[3 * (p) + 1] + nnel[1] : 0); yy++) \
+        for (int64_t xx = ec[3 * (p)] - nnel[0]; xx <= ec[3 * (p)] + nnel[0]; xx++)               \
+            for (int64_t zz = ec[3 * (p) + zl] - nnel[zl]; zz <= ec[3 * (p) + zl] + nnel[zl]; zz++) { \
+                if (xx < 0 || xx >= nel[0] || zz < 0 || zz >= nel[zl] || (dim == 3 && (yy < 0 || yy >= nel[1]))) continue; \
+                const int64_t el = EL_ID(xx, yy, zz);                                              \
+                BODY                                                                               \
+            }
+    /* "tplgy": e2p[p, el] = p for el in p's neighbourhood -> per-element lists, p ascending */
+    for (int64_t p = 0; p < nmp; p++) { NBR_LOOP(p, cnt[el + 1]++;) }
+    for (int64_t e = 0; e < ntot; e++) cnt[e + 1] += cnt[e];
+    int64_t *lst = malloc(sizeof(int64_t) * (size_t)(cnt[ntot] ? cnt[ntot] : 1));
+    int64_t *fill = malloc(sizeof(int64_t) * (size_t)ntot);
+    if (!lst || !fill) { free(ec); free(cnt); free(W); free(lst); free(fill); return -1; }
+    memcpy(fill, cnt, sizeof(int64_t) * (size_t)ntot);
+    for (int64_t p = 0; p < nmp; p++) { NBR_LOOP(p, lst[fill[el]++] = p;) }
+#undef NBR_LOOP
+#undef EL_ID
+    /* pair map and p2p columns sized by the plastic particles' candidates */
+    int64_t npair = 0;
+    for (int64_t p = 0; p < nmp; p++)
+        if (mp->dlam[p] != 0.0) { const int64_t e = mp->p2e[p] - 1; npair += cnt[e + 1] - cnt[e]; }
+    pairmap m;
+    m.cap = 16;
+    while (m.cap < 2 * npair + 16) m.cap *= 2;
+    m.key = malloc(sizeof(int64_t) * (size_t)m.cap);
+    m.val = malloc(sizeof(double) * (size_t)m.cap);
+    int64_t *p2p = malloc(sizeof(int64_t) * (size_t)(npair ? npair : 1));
+    int64_t *p2p0 = malloc(sizeof(int64_t) * (size_t)(nmp + 1));
+    if (!m.key || !m.val || !p2p || !p2p0) {
+        free(ec); free(cnt); free(W); free(lst); free(fill); free(m.key); free(m.val); free(p2p); free(p2p0);
+        return -1;
+    }
+    for (int64_t i = 0; i < m.cap; i++) { m.key[i] = -1; m.val[i] = 0.0; }
+    for (int64_t p = 0; p < nmp; p++) mp->epII[1 + 2 * p] = 0.0;
+    /* "p->q" */
+    int64_t np2p = 0;
+    for (int64_t p = 0; p < nmp; p++) {
+        p2p0[p] = np2p;
+        if (mp->dlam[p] == 0.0) continue;
+        const int64_t e = mp->p2e[p] - 1;
+        for (int64_t k = cnt[e]; k < cnt[e + 1]; k++) {
+            const int64_t q = lst[k];
+            const double xi = mp->x[0 + dim * p] - mp->x[0 + dim * q];
+            const double eta = mp->x[1 + dim * p] - mp->x[1 + dim * q];
+            const double d = sqrt(xi * xi + eta * eta);
+            const int64_t s = pm_slot(&m, p, q);
+            if (m.val[s] == 0.0) {
+                const double w0 = d / in->ls * exp(-((d / in->ls) * (d / in->ls)));
+                m.val[s] = w0;
+                W[p] += w0;
+                W[q] += w0;
+                p2p[np2p++] = q;
+            }
+        }
+    }
+    p2p0[nmp] = np2p;
+    /* "p<-q" */
+    for (int64_t p = 0; p < nmp; p++) {
+        if (mp->dlam[p] == 0.0) continue;
+        if (!(W[p] > 1e-16)) {
+            mp->epII[1 + 2 * p] = mp->epII[0 + 2 * p];
+        } else {
+            for (int64_t k = p2p0[p]; k < p2p0[p + 1]; k++) {
+                const int64_t q = p2p[k];
+                mp->epII[1 + 2 * p] += (m.val[pm_slot(&m, p, q)] / W[p]) * mp->epII[0 + 2 * q];
+            }
+        }
+    }
+    free(ec); free(cnt); free(W); free(lst); free(fill); free(m.key); free(m.val); free(p2p); free(p2p0);
+    return 0;
+}
+
 void or_plast(or_point *mp, const or_instr *in) {
     const int dim = mp->dim, nstr = dim == 2 ? 3 : 6;
     const int64_t nmp = mp->nmp;
-    for (int64_t p = 0; p < nmp; p++) mp->epII[1 + 2 * p] = mp->epII[0 + 2 * p];
+    /* without nonlocal regularisation ϵpII[2,:] = ϵpII[1,:] (plast.jl:61-63);
+     * with it, or_step has run or_nonlocal first */
+    if (!in->nonloc)
+        for (int64_t p = 0; p < nmp; p++) mp->epII[1 + 2 * p] = mp->epII[0 + 2 * p];
     PAR_FOR
     for (int64_t p = 0; p < nmp; p++) {
         double *s = in->deform == OR_FINITE ? mp->tau + nstr * p : mp->sig + nstr * p;
@@ -937,7 +1060,10 @@ int or_step(or_point *mp, or_mesh *me, const or_instr *in, const double *g, doub
     or_n2p(mp, me, dt, in);
     or_update(mp, me, dt, in);
     or_elast(mp, in);
-    if (with_plast && in->plast != OR_PLAST_NONE) or_plast(mp, in);
+    if (with_plast && in->plast != OR_PLAST_NONE) {
+        if (in->nonloc && or_nonlocal(mp, me, in) != 0) return -1;
+        or_plast(mp, in);
+    }
     return OR_OK;
 }
 

@@ -80,6 +80,8 @@ typedef struct or_instr {
     double C_pf, damping;
     double Kc, Gc, Hp, c;
     double Del[36];          /* [nstr,nstr] column-major */
+    int32_t nonloc, pad1;    /* instr.nonloc.status */
+    double ls;               /* instr.nonloc.ls */
 } or_instr;
 
 int  or_version(void);
@@ -100,6 +102,7 @@ void or_n2p(or_point *mp, or_mesh *me, double dt, const or_instr *in);
 void or_update(or_point *mp, or_mesh *me, double dt, const or_instr *in);
 void or_elast(or_point *mp, const or_instr *in);
 void or_plast(or_point *mp, const or_instr *in);
+int  or_nonlocal(or_point *mp, const or_mesh *me, const or_instr *in);
 
 /* One step of elastoplastic!/elastodynamic! (src/home/program/workflow/
  * worflow.jl:30-38, 76-84) after dt and g were chosen. with_plast=0 is

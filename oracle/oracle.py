@@ -42,7 +42,8 @@ class OrInstr(C.Structure):
     _fields_ = [("basis", C.c_int32), ("trsfr", C.c_int32), ("musl", C.c_int32), ("deform", C.c_int32),
                 ("plast", C.c_int32), ("locking", C.c_int32), ("nthreads", C.c_int32), ("domain", C.c_int32),
                 ("C_pf", C.c_double), ("damping", C.c_double), ("Kc", C.c_double), ("Gc", C.c_double),
-                ("Hp", C.c_double), ("c", C.c_double), ("Del", C.c_double * 36)]
+                ("Hp", C.c_double), ("c", C.c_double), ("Del", C.c_double * 36),
+                ("nonloc", C.c_int32), ("pad1", C.c_int32), ("ls", C.c_double)]
 
 
 _lib = None
@@ -71,6 +72,7 @@ def load():
         L.or_update.argtypes = [C.POINTER(OrPoint), C.POINTER(OrMesh), C.c_double, C.POINTER(OrInstr)]
         L.or_elast.argtypes = [C.POINTER(OrPoint), C.POINTER(OrInstr)]
         L.or_plast.argtypes = [C.POINTER(OrPoint), C.POINTER(OrInstr)]
+        L.or_nonlocal.argtypes = [C.POINTER(OrPoint), C.POINTER(OrMesh), C.POINTER(OrInstr)]
         L.or_set_threads.argtypes = [C.c_int]
         L.or_bsmpm_1d.argtypes = [C.c_double, C.c_int, _dp, _dp]
         L.or_gimp_1d.argtypes = [C.c_double, C.c_double, C.c_double, _dp, _dp]
@@ -146,6 +148,9 @@ class Oracle:
         ins.plast, ins.locking, ins.nthreads = ops["plast"], ops["locking"], threads
         ins.domain = ops.get("domain", 0)
         ins.C_pf, ins.damping = ops["C_pf"], ops["damping"]
+        nl = instr.get("nonloc", {})
+        ins.nonloc = 1 if nl.get("status", False) else 0
+        ins.ls = float(np.max(nl.get("ls", 0.5)))
         ins.Kc, ins.Gc, ins.Hp, ins.c = cmpr.Kc, cmpr.Gc, cmpr.Hp, cmpr.c
         for k, v in enumerate(np.asarray(cmpr.Del).ravel(order="F")):
             ins.Del[k] = v

@@ -260,3 +260,27 @@ def test_resume_through_host_state(dim):
     ref = clone(ic)
     Oracle(ref.mesh, ref.mpts, ref.cmpr, cfg.instr).run(30, 100.0)
     assert max(compare(two.mpts, ref.mpts, fields).values()) <= RUN_TOL
+
+
+@pytest.mark.parametrize("dim,nsteps,ls", [(2, 80, 2.0), (2, 40, 5.0), (3, 25, 2.0)])
+def test_nonlocal_regularisation(dim, nsteps, ls):
+    """Non-local plastic strain (nonlocal.jl:18-47, plast.jl:49-60) with the
+    reference's softening material (c0 20 kPa -> cr 4 kPa, Hp -30 kPa, weak
+    layer φr 7.5°), so that ϵpII[2] enters the cohesion of the return."""
+    ic, cfg = slump(dim, C_pf=1.0)
+    cfg = clone(cfg)
+    cfg.instr["nonloc"] = {"status": True, "ls": ls}
+    x0 = ic.mpts.x.copy(order="F")
+    ref, dev = clone(ic), clone(ic)
+    Oracle(ref.mesh, ref.mpts, ref.cmpr, cfg.instr).run(nsteps, 100.0)
+    r, _ = _device_run(dev, cfg, nsteps, 100.0)
+    assert r.steps == nsteps
+    a, b = dev.mpts, ref.mpts
+    npl = int((b.s.dlam != 0).sum())
+    e1, e2 = relerr(a.s.epII[0], b.s.epII[0]), relerr(a.s.epII[1], b.s.epII[1])
+    du = relerr(a.x - x0, b.x - x0)
+    dtau = relerr(a.s.tau, b.s.tau)
+    print(f"non-local {dim}D ls={ls} {nsteps} steps: plastic {npl}, epII1 {e1:.2e}, epII2 {e2:.2e}, "
+          f"displacement {du:.2e}, stress {dtau:.2e}")
+    assert npl > 0 and np.abs(b.s.epII[1]).max() > 0
+    assert max(e1, e2, du, dtau) <= RUN_TOL
