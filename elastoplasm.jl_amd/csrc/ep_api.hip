@@ -802,7 +802,8 @@ static void launch_step(ep_ctx *c, const PState<T> &P) {
     const Mat M = c->mat;
     timed(c, K_BEGIN, [&] { EP_LAUNCH(k_begin<D>, 1, 64, s, ct, g, M); });
     timed(c, K_ZERO, [&] { EP_LAUNCH(k_zero_grid<D>, gg, TB, s, ct, G); });
-    timed(c, K_P2G, [&] { EP_LAUNCH((k_p2g<D, B, T>), gp, TB, s, ct, g, P, G, M); });
+    if (M.tpic || M.apic) timed(c, K_P2G, [&] { EP_LAUNCH((k_p2g<D, B, T, true>), gp, TB, s, ct, g, P, G, M); });
+    else timed(c, K_P2G, [&] { EP_LAUNCH((k_p2g<D, B, T, false>), gp, TB, s, ct, g, P, G, M); });
     timed(c, K_EULER, [&] { EP_LAUNCH(k_euler<D>, gg, TB, s, ct, G, M); });
     timed(c, K_G2P, [&] { EP_LAUNCH((k_g2p<D, B, T>), gp, TB, s, ct, g, P, G, M); });
     if (M.apic) timed(c, K_BIJ, [&] { EP_LAUNCH((k_bij<D, B, T>), gp, TB, s, ct, g, P, G); });
@@ -1449,7 +1450,8 @@ int ens_capture(ep_ensemble *e) {
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     EP_LAUNCH((kb_begin<D, T>), (n + 63) / 64, 64, s, I, n);
     EP_LAUNCH((kb_zero_grid<D, T>), e->ngblk, tb, s, I, gm);
-    EP_LAUNCH((kb_p2g<D, B, T>), e->npblk, tb, s, I, pm);
+    if (M.tpic || M.apic) EP_LAUNCH((kb_p2g<D, B, T, true>), e->npblk, tb, s, I, pm);
+    else EP_LAUNCH((kb_p2g<D, B, T, false>), e->npblk, tb, s, I, pm);
     EP_LAUNCH((kb_euler<D, T>), e->ngblk, tb, s, I, gm);
     EP_LAUNCH((kb_g2p<D, B, T>), e->npblk, tb, s, I, pm);
     if (M.apic) EP_LAUNCH((kb_bij<D, B, T>), e->npblk, tb, s, I, pm);

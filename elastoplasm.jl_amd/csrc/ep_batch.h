@@ -55,11 +55,11 @@ __global__ void kb_augv(const Inst<T> *I, BlockMap m) {
     b_augv<D>(I[i].c, I[i].G, long(blockIdx.x) - m.first[i]);
 }
 
-template <int D, int B, typename T>
+template <int D, int B, typename T, bool AFF>
 __global__ void __launch_bounds__(EP_PBLK_MAX) kb_p2g(const Inst<T> *I, BlockMap m) {
     const int i = m.inst[blockIdx.x];
     const Inst<T> &in = I[i];
-    b_p2g<D, B, T>(in.c, in.g, in.P, in.G, in.M, long(blockIdx.x) - m.first[i]);
+    b_p2g<D, B, T, AFF>(in.c, in.g, in.P, in.G, in.M, long(blockIdx.x) - m.first[i]);
 }
 
 template <int D, int B, typename T>

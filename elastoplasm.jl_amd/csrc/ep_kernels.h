@@ -305,16 +305,20 @@ struct BlockBox {
         }
         __syncthreads();
         long n = 1;
-        bool ok = sint[0] != 0x7fffffff;
+        // the box is block-uniform: keep it in scalar registers
+        int mn[D], mx[D];
+#pragma unroll
+        for (int i = 0; i < D; i++) { mn[i] = uniform(sint[i]); mx[i] = uniform(sint[D + i]); }
+        bool ok = mn[0] != 0x7fffffff;
 #pragma unroll
         for (int i = 0; i < D; i++) {
-            lo[i] = sint[i];
-            const long e = long(sint[D + i]) - long(sint[i]) + 4;
+            lo[i] = mn[i];
+            const long e = long(mx[i]) - long(mn[i]) + 4;
             ok = ok && e >= 4 && e <= cap;
             ext[i] = ok ? int(e) : 4;
             n *= ext[i];
         }
-        any = sint[0] != 0x7fffffff;
+        any = mn[0] != 0x7fffffff;
         nloc = (ok && n <= cap) ? int(n) : 0;
         nz4 = (ext[D - 1] + 3) >> 2;
         nx4 = (ext[0] + 3) >> 2;
@@ -523,17 +527,21 @@ __device__ __forceinline__ void for_stencil_box(const BlockBox<D> &b, const W<D,
         }
     } else {
         const int sy = 16 * NV * b.nz4 * b.nx4;
-        auto plane = [&](int k0) {
-            const int k = ROT ? ((k0 + ry) & 3) : k0;
-            const T wy = ROT ? sel4(w.w[1], k) : w.w[1][k0];
-            const T dwy = ROT ? sel4(w.dw[1], k) : w.dw[1][k0];
-            const int ay = sy * (yb + k);
-#pragma unroll
-            for (int i0 = 0; i0 < 4; i0++) {
-                const T wxy = wx[i0] * wy;
-                const T dxy = dwx[i0] * wy;
-                const T xdy = wx[i0] * dwy;
-                const int row = ay + ax[i0];
+        if constexpr (ROT && NV >= 5) {
+            // many-field scatters (P2G): one (y, x) row of 4 nodes per
+            // iteration, the row's weights picked with selects; rolling the 16
+            // rows keeps the register count down (FP64 P2G 225 -> 148 VGPRs,
+            // 3 waves per SIMD instead of 2: 1603 -> 1210 us)
+#pragma unroll 1
+            for (int m = 0; m < 16; m++) {
+                const int k = ((m >> 2) + ry) & 3, i = ((m & 3) + rx) & 3;
+                const T wy = sel4(w.w[1], k), dwy = sel4(w.dw[1], k);
+                const T wxv = sel4(w.w[0], i), dwxv = sel4(w.dw[0], i);
+                const int x = xb + i;
+                const int row = sy * (yb + k) + 16 * NV * b.nz4 * (x >> 2) + 4 * (x & 3);
+                const T wxy = wxv * wy;
+                const T dxy = dwxv * wy;
+                const T xdy = wxv * dwy;
 #pragma unroll
                 for (int j0 = 0; j0 < 4; j0++) {
                     T dN[D];
@@ -541,19 +549,35 @@ __device__ __forceinline__ void for_stencil_box(const BlockBox<D> &b, const W<D,
                     dN[0] = dxy * wz[j0];
                     dN[1] = xdy * wz[j0];
                     dN[2] = wxy * dwz[j0];
-                    const int ix[3] = {w.base[0] + ox[i0], w.base[1] + k, w.base[2] + oz[j0]};
+                    const int ix[3] = {w.base[0] + i, w.base[1] + k, w.base[2] + oz[j0]};
                     fn(0L, row + az[j0], N, dN, ix);
                 }
             }
-        };
-        if constexpr (ROT) {
-            // scatters: one y plane (16 nodes) per iteration keeps the
-            // register count of the 7-field P2G scatter down
-#pragma unroll 1
-            for (int k0 = 0; k0 < 4; k0++) plane(k0);
         } else {
 #pragma unroll
-            for (int k0 = 0; k0 < 4; k0++) plane(k0);
+            for (int k0 = 0; k0 < 4; k0++) {
+                const int k = ROT ? ((k0 + ry) & 3) : k0;
+                const T wy = ROT ? sel4(w.w[1], k) : w.w[1][k0];
+                const T dwy = ROT ? sel4(w.dw[1], k) : w.dw[1][k0];
+                const int ay = sy * (yb + k);
+#pragma unroll
+                for (int i0 = 0; i0 < 4; i0++) {
+                    const T wxy = wx[i0] * wy;
+                    const T dxy = dwx[i0] * wy;
+                    const T xdy = wx[i0] * dwy;
+                    const int row = ay + ax[i0];
+#pragma unroll
+                    for (int j0 = 0; j0 < 4; j0++) {
+                        T dN[D];
+                        const T N = wxy * wz[j0];
+                        dN[0] = dxy * wz[j0];
+                        dN[1] = xdy * wz[j0];
+                        dN[2] = wxy * dwz[j0];
+                        const int ix[3] = {w.base[0] + ox[i0], w.base[1] + k, w.base[2] + oz[j0]};
+                        fn(0L, row + az[j0], N, dN, ix);
+                    }
+                }
+            }
         }
     }
 }
@@ -734,7 +758,9 @@ __device__ __forceinline__ void apic_affine(const Geom &g, const W<D, T> &w, con
 }
 
 // transform + std/tpic/apic_{2,3}d_p2n (p2n.jl:12-21, 58-102, 140-191, 231-280)
-template <int D, int B, typename T>
+// AFF: the affine momentum term of TPIC / APIC, its own instantiation so that
+// the standard transfer carries no ∇v registers through its scatter
+template <int D, int B, typename T, bool AFF>
 __device__ __forceinline__ void b_p2g(const Ctrl *c, const Geom &g, const PState<T> &P, const GState &G, const Mat &M, long blk) {
     constexpr int NS = D == 2 ? 3 : 6;
     constexpr int NV = 1 + 2 * D;
@@ -768,21 +794,23 @@ __device__ __forceinline__ void b_p2g(const Ctrl *c, const Geom &g, const PState
     T v[D], pm[D], gv[D][D];
 #pragma unroll
     for (int i = 0; i < D; i++) { v[i] = P.v[i][q]; pm[i] = ms * v[i]; }
-    if (M.tpic) {
+    if constexpr (AFF) {
+        if (M.tpic) {
 #pragma unroll
-        for (int i = 0; i < D; i++)
+            for (int i = 0; i < D; i++)
 #pragma unroll
-            for (int j = 0; j < D; j++) gv[i][j] = P.gv[i + D * j][q];
+                for (int j = 0; j < D; j++) gv[i][j] = P.gv[i + D * j][q];
+        }
+        // APIC scatters TPIC's form with B·D⁻¹ in place of ∇v
+        if (M.apic) apic_affine<D, T>(g, w, xp, P, q, gv);
     }
-    // APIC scatters TPIC's form with B·D⁻¹ in place of ∇v
-    if (M.apic) apic_affine<D, T>(g, w, xp, P, q, gv);
     const T gz = T(c->g[D - 1]);
-    const bool tpic = M.tpic || M.apic;
+    constexpr bool tpic = AFF;
     auto scatter = [&](const auto &sink) {
     scatter_walk<D, T>(g, w, sink, [&](long no, int li, T N, const T *dN, const int *ix) {
         sink(0, no, li, double(N * ms));
         if (D == 2) {
-            if (tpic) {
+            if constexpr (tpic) {
                 // Δnp = x_n - x_p (misc.jl:1-14)
                 const T dx = T(g.X[0][clampi(ix[0], g.nno[0])] - xp[0]);
                 const T dy = T(g.X[1][clampi(ix[1], g.nno[1])] - xp[1]);
@@ -795,7 +823,7 @@ __device__ __forceinline__ void b_p2g(const Ctrl *c, const Geom &g, const PState
             sink(3, no, li, double(-(Om * (dN[0] * s[0] + dN[1] * s[2]))));
             sink(4, no, li, double(-(Om * (dN[0] * s[2] + dN[1] * s[1]) - N * (ms * gz))));
         } else {
-            if (tpic) {
+            if constexpr (tpic) {
                 const T dx = T(g.X[0][clampi(ix[0], g.nno[0])] - xp[0]);
                 const T dy = T(g.X[1][clampi(ix[1], g.nno[1])] - xp[1]);
                 const T dz = T(g.X[2][clampi(ix[2], g.nno[2])] - xp[2]);
@@ -834,8 +862,8 @@ __device__ __forceinline__ void b_p2g(const Ctrl *c, const Geom &g, const PState
 }
 
 // kernel of one context: block b of the launch is block b of the particles / nodes
-template <int D, int B, typename T>
-__global__ void __launch_bounds__(EP_PBLK_MAX) k_p2g(const Ctrl *c, Geom g, PState<T> P, GState G, Mat M) { b_p2g<D, B, T>(c, g, P, G, M, blockIdx.x); }
+template <int D, int B, typename T, bool AFF>
+__global__ void __launch_bounds__(EP_PBLK_MAX) k_p2g(const Ctrl *c, Geom g, PState<T> P, GState G, Mat M) { b_p2g<D, B, T, AFF>(c, g, P, G, M, blockIdx.x); }
 
 // euler (solve.jl:14-40); a and v of massless nodes are the zeros that
 // solve() writes first (solve.jl:58-59).

@@ -8,6 +8,7 @@
 #define EP_LDS
 inline void lds_atomic_add(double *p, double v) { std::atomic_ref<double>(*p).fetch_add(v); }
 #define EP_MFMA_32X32X8_BF16(a, b, c) emu_mfma_32x32x8bf16(a, b, c)
+inline int uniform(int v) { return v; }
 #else
 #include <hip/hip_runtime.h>
 #define EP_BLOCK 256
@@ -18,6 +19,8 @@ __device__ __forceinline__ void lds_atomic_add(EP_LDS double *p, double v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 #define EP_LAUNCH(kern, grid, block, stream, ...) hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, stream, __VA_ARGS__)
+// a value every lane of the wave holds alike, moved to a scalar register
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 // MFMA operand types: 4 bf16 (as raw 16-bit patterns) and 16 FP32 accumulators
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v16f __attribute__((ext_vector_type(16)));
