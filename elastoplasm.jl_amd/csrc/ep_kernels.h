@@ -447,17 +447,20 @@ __device__ __forceinline__ void for_stencil_rot(const Geom &g, const W<D, T> &w,
         }
     } else {
         const int rz = lane & 3, rx = (lane >> 2) & 3;
-#pragma unroll
+        // the fallback of blocks whose box does not fit: rolled, so that it
+        // does not raise the register count of the kernel's box path
+#pragma unroll 1
         for (int k = 0; k < 4; k++) {
             if (!(w.ok >> (4 + k) & 1u)) continue;
-#pragma unroll
+#pragma unroll 1
             for (int i0 = 0; i0 < 4; i0++) {
                 const int i = (i0 + rx) & 3;
                 if (!(w.ok >> i & 1u)) continue;
                 const T wx = sel4(w.w[0], i), dwx = sel4(w.dw[0], i);
-                const T wxy = wx * w.w[1][k];
-                const T dxy = dwx * w.w[1][k];
-                const T xdy = wx * w.dw[1][k];
+                const T wy = sel4(w.w[1], k), dwy = sel4(w.dw[1], k);
+                const T wxy = wx * wy;
+                const T dxy = dwx * wy;
+                const T xdy = wx * dwy;
 #pragma unroll
                 for (int j0 = 0; j0 < 4; j0++) {
                     const int j = (j0 + rz) & 3;
@@ -806,6 +809,7 @@ __device__ __forceinline__ void b_p2g(const Ctrl *c, const Geom &g, const PState
     }
     const T gz = T(c->g[D - 1]);
     constexpr bool tpic = AFF;
+    const T mgz = ms * gz;
     auto scatter = [&](const auto &sink) {
     scatter_walk<D, T>(g, w, sink, [&](long no, int li, T N, const T *dN, const int *ix) {
         sink(0, no, li, double(N * ms));
@@ -821,7 +825,7 @@ __device__ __forceinline__ void b_p2g(const Ctrl *c, const Geom &g, const PState
                 sink(2, no, li, double(N * pm[1]));
             }
             sink(3, no, li, double(-(Om * (dN[0] * s[0] + dN[1] * s[2]))));
-            sink(4, no, li, double(-(Om * (dN[0] * s[2] + dN[1] * s[1]) - N * (ms * gz))));
+            sink(4, no, li, double(-(Om * (dN[0] * s[2] + dN[1] * s[1]) - N * mgz)));
         } else {
             if constexpr (tpic) {
                 const T dx = T(g.X[0][clampi(ix[0], g.nno[0])] - xp[0]);
@@ -838,7 +842,7 @@ __device__ __forceinline__ void b_p2g(const Ctrl *c, const Geom &g, const PState
             // Voigt [xx,yy,zz,yz,xz,xy]: σyx = s[5], σzy = s[3], σzx = s[4]
             sink(4, no, li, double(-(Om * (dN[0] * s[0] + dN[1] * s[5] + dN[2] * s[4]))));
             sink(5, no, li, double(-(Om * (dN[0] * s[5] + dN[1] * s[1] + dN[2] * s[3]))));
-            sink(6, no, li, double(-(Om * (dN[0] * s[4] + dN[1] * s[3] + dN[2] * s[2]) - N * (ms * gz))));
+            sink(6, no, li, double(-(Om * (dN[0] * s[4] + dN[1] * s[3] + dN[2] * s[2]) - N * mgz)));
         }
     });
     };
