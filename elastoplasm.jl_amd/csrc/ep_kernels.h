@@ -273,6 +273,14 @@ constexpr int ACC_LDS = 4096;   // doubles of LDS per block (32 KiB)
 // P2G's box (7 fields in 3D): 39.9 KiB, so that 4 blocks still share a CU's
 // 160 KiB as with ACC_LDS (occupancy is LDS-bound either way)
 constexpr int ACC_LDS_P2G = 5104;
+// G2P's box (6 fields staged, then 3 scattered): 24 KiB, so that 6 blocks
+// share a CU (its 80 VGPRs allow 6 waves per SIMD)
+constexpr int ACC_LDS_G2P = 3072;
+// deform (3 fields staged, 1 scattered): 16 KiB; F-bar (4 fields): 24 KiB.
+// Blocks per CU are then bounded by registers, not LDS (FP32 deform 64
+// VGPRs: 8 waves per SIMD; FP32 F-bar 99: 5)
+constexpr int ACC_LDS_DEFORM = 2048;
+constexpr int ACC_LDS_FBAR = 3072;
 // Box layout.  The LDS serves a 64-bit access as four quarter-waves of 16
 // lanes, each through 32 four-byte banks, i.e. 16 bank pairs per double; the
 // measured bank-conflict counts of the walks match that model exactly.
@@ -914,7 +922,7 @@ __global__ void k_euler(const Ctrl *c, GState G, Mat M) { b_euler<D>(c, G, M, bl
 // picflip_n2p + augm_momentum (n2p.jl:14-40, augm.jl:13-25)
 template <int D, int B, typename T>
 __device__ __forceinline__ void b_g2p(Ctrl *c, const Geom &g, const PState<T> &P, const GState &G, const Mat &M, long blk) {
-    __shared__ double smem[ACC_LDS];
+    __shared__ double smem[ACC_LDS_G2P];
     __shared__ int sint[8];
     __shared__ unsigned long long sred[16];
     if (!c->active) return;
@@ -929,7 +937,7 @@ __device__ __forceinline__ void b_g2p(Ctrl *c, const Geom &g, const PState<T> &P
     W<D, T> w;
     weights<D, B, T>(g, xp, lp, w);
     BlockBox<D> box;
-    box.begin((EP_LDS double *)smem, sint, live, w.base);
+    box.begin((EP_LDS double *)smem, sint, live, w.base, ACC_LDS_G2P);
     if (!live) w.retire(box.lo);
     const bool staged = box.fits(2 * D);
     if (staged) {
@@ -1516,7 +1524,7 @@ __device__ __forceinline__ void domain_update(const PState<T> &P, const Mat &M, 
 //   written back, k_fbar recomputes ΔF from the same grid velocity.
 template <int D, int B, typename T, bool LOCK, bool SURR = false>
 __device__ __forceinline__ void b_deform(Ctrl *c, const Geom &g, const PState<T> &P, const GState &G, const Mat &M, long blk) {
-    __shared__ double smem[ACC_LDS];
+    __shared__ double smem[ACC_LDS_DEFORM];
     __shared__ int sint[8];
     if (!c->active) return;
     const long p = blk * long(blockDim.x) + threadIdx.x;
@@ -1530,7 +1538,7 @@ __device__ __forceinline__ void b_deform(Ctrl *c, const Geom &g, const PState<T>
     W<D, T> w;
     weights<D, B, T>(g, xp, lp, w);
     BlockBox<D> box;
-    box.begin((EP_LDS double *)smem, sint, live, w.base);
+    box.begin((EP_LDS double *)smem, sint, live, w.base, ACC_LDS_DEFORM);
     if (!live) w.retire(box.lo);
     const bool staged = box.fits(D);
     if (staged) box.template stage<D>(g, G.v);
@@ -1586,7 +1594,7 @@ __global__ void __launch_bounds__(EP_PBLK_MAX) k_deform(Ctrl *c, Geom g, PState<
 // (Σ N ΔJ_n/ΔJ_p)^(1/d) before the constitutive update.
 template <int D, int B, typename T, bool SURR = false>
 __device__ __forceinline__ void b_fbar(Ctrl *c, const Geom &g, const PState<T> &P, const GState &G, const Mat &M, long blk) {
-    __shared__ double smem[ACC_LDS];
+    __shared__ double smem[ACC_LDS_FBAR];
     __shared__ int sint[8];
     if (!c->active) return;
     const long p = blk * long(blockDim.x) + threadIdx.x;
@@ -1600,7 +1608,7 @@ __device__ __forceinline__ void b_fbar(Ctrl *c, const Geom &g, const PState<T> &
     W<D, T> w;
     weights<D, B, T>(g, xp, lp, w);
     BlockBox<D> box;
-    box.begin((EP_LDS double *)smem, sint, live, w.base);
+    box.begin((EP_LDS double *)smem, sint, live, w.base, ACC_LDS_FBAR);
     if (!live) w.retire(box.lo);
     const bool staged = box.fits(D + 1);
     if (staged) {
