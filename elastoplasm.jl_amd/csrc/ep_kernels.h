@@ -420,9 +420,16 @@ struct GlobSrc {
 
 __device__ __forceinline__ int clampi(int i, int n) { return i < 0 ? 0 : (i >= n ? n - 1 : i); }
 
+// a[i & 3] from registers.  Written as two bit tests so that it lowers to
+// v_cndmask selects: the compare chain i == 0 / 1 / 2 was turned into a
+// switch and lowered as divergent branches (about 20 instructions, 12 of them
+// scalar, per select).
 template <typename T>
 __device__ __forceinline__ T sel4(const T *a, int i) {
-    return i == 0 ? a[0] : (i == 1 ? a[1] : (i == 2 ? a[2] : a[3]));
+    const bool b0 = (i & 1) != 0, b1 = (i & 2) != 0;
+    const T lo = b0 ? a[1] : a[0];
+    const T hi = b0 ? a[3] : a[2];
+    return b1 ? hi : lo;
 }
 
 // Scatter iteration over the 4^d stencil with a lane-dependent rotation of
