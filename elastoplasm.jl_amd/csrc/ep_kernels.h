@@ -600,6 +600,40 @@ __device__ __forceinline__ void for_stencil_box(const BlockBox<D> &b, const W<D,
     }
 }
 
+// The 3D box scatter by rows: fn(row, wxy, dxy, xdy) once per (y, x) row of
+// the stencil, in for_stencil_box's bank-aware order (ROT); the row's 4 z
+// nodes sit at row + z.a[j0] with weights z.w[j0], derivatives z.dw[j0].
+// A scatter whose nodal values are linear in (N, ∂N/∂z) for fixed row
+// factors (P2G's m, p and f) then spends two products per value and node.
+template <typename T>
+struct ZNodes {
+    T w[4], dw[4];
+    int a[4];
+};
+template <typename T, int NV, typename Fn>
+__device__ __forceinline__ void box_rows_3d(const BlockBox<3> &b, const W<3, T> &w, ZNodes<T> &z, Fn &&fn) {
+    const int lane = threadIdx.x & 63;
+    const int zb = w.base[2] - b.lo[2], xb = w.base[0] - b.lo[0], yb = w.base[1] - b.lo[1];
+    const int rz = (lane - zb) & 3, rx = ((lane >> 2) - xb) & 3, ry = (lane >> 4) & 3;
+#pragma unroll
+    for (int j0 = 0; j0 < 4; j0++) {
+        const int j = (j0 + rz) & 3;
+        z.w[j0] = sel4(w.w[2], j);
+        z.dw[j0] = sel4(w.dw[2], j);
+        const int zz = zb + j;
+        z.a[j0] = 16 * NV * (zz >> 2) + (zz & 3);
+    }
+    const int sy = 16 * NV * b.nz4 * b.nx4;
+#pragma unroll 1
+    for (int m = 0; m < 16; m++) {
+        const int k = ((m >> 2) + ry) & 3, i = ((m & 3) + rx) & 3;
+        const T wy = sel4(w.w[1], k), dwy = sel4(w.dw[1], k);
+        const T wxv = sel4(w.w[0], i), dwxv = sel4(w.dw[0], i);
+        const int x = xb + i;
+        fn(sy * (yb + k) + 16 * NV * b.nz4 * (x >> 2) + 4 * (x & 3), wxv * wy, dwxv * wy, wxv * dwy);
+    }
+}
+
 // Walks for the two sinks / sources; fn(node, box index, N, dN, axis
 // indices).  On the box walks the axis indices may lie outside the grid
 // (zero weight): clamp them before reading node coordinates.
@@ -825,6 +859,41 @@ __device__ __forceinline__ void b_p2g(const Ctrl *c, const Geom &g, const PState
     const T gz = T(c->g[D - 1]);
     constexpr bool tpic = AFF;
     const T mgz = ms * gz;
+    // 3D standard transfer into the LDS box, by stencil rows: with
+    // so = -Ω σ and N = wxy wz, ∂N = (dxy wz, xdy wz, wxy dwz),
+    //   m N = (m wxy) wz,  p_i N = (p_i wxy) wz,
+    //   f_x = (so_xx dxy + so_xy xdy) wz + (so_xz wxy) dwz, likewise y,
+    //   f_z = (so_xz dxy + so_yz xdy + m g_z wxy) wz + (so_zz wxy) dwz,
+    // the same sums as the per-node form below up to rounding.  FP64 only:
+    // FP64 p2g 1138 -> 1105 us, while in FP32 the row factors raise the
+    // kernel from 89 to 132 VGPRs (5 -> 3 waves per SIMD, 949 -> 1116 us).
+    constexpr bool rows = D == 3 && !AFF && sizeof(T) == 8;
+    auto scatter_rows = [&](const LdsSink<D, NV> &sink) {
+        if constexpr (rows) {
+            T so[NS];
+#pragma unroll
+            for (int k = 0; k < NS; k++) so[k] = -(Om * s[k]);
+            ZNodes<T> z;
+            box_rows_3d<T, NV>(*sink.b, w, z, [&](int row, T wxy, T dxy, T xdy) {
+                const T mw = ms * wxy, p0 = pm[0] * wxy, p1 = pm[1] * wxy, p2 = pm[2] * wxy;
+                const T ax = so[0] * dxy + so[5] * xdy, bx = so[4] * wxy;
+                const T ay = so[5] * dxy + so[1] * xdy, by = so[3] * wxy;
+                const T az = so[4] * dxy + so[3] * xdy + mgz * wxy, bz = so[2] * wxy;
+#pragma unroll
+                for (int j0 = 0; j0 < 4; j0++) {
+                    const int li = row + z.a[j0];
+                    const T wz = z.w[j0], dwz = z.dw[j0];
+                    sink(0, 0, li, double(mw * wz));
+                    sink(1, 0, li, double(p0 * wz));
+                    sink(2, 0, li, double(p1 * wz));
+                    sink(3, 0, li, double(p2 * wz));
+                    sink(4, 0, li, double(ax * wz + bx * dwz));
+                    sink(5, 0, li, double(ay * wz + by * dwz));
+                    sink(6, 0, li, double(az * wz + bz * dwz));
+                }
+            });
+        }
+    };
     auto scatter = [&](const auto &sink) {
     scatter_walk<D, T>(g, w, sink, [&](long no, int li, T N, const T *dN, const int *ix) {
         sink(0, no, li, double(N * ms));
@@ -873,7 +942,8 @@ __device__ __forceinline__ void b_p2g(const Ctrl *c, const Geom &g, const PState
         if (sl) box.zero(NV);
         if (mine) {
             if (!live) w.retire(box.lo);
-            if (sl) scatter(LdsSink<D, NV>{&box});
+            if (rows && sl) scatter_rows(LdsSink<D, NV>{&box});
+            else if (sl) scatter(LdsSink<D, NV>{&box});
             else scatter(GlobSink{dst});
         }
         if (sl) box.template flush<NV>(g, dst);
@@ -1449,6 +1519,55 @@ __device__ __forceinline__ void grad_v(const Geom &g, const W<D, T> &w, const Sr
     });
 }
 
+// ∇v gather from a staged 3D box by stencil rows (y outer, x, z inner, the
+// reference's node order): with N = wxy wz,
+//   Σ v_n ∂N/∂x = Σ_rows dxy Σ_z wz v_n,  ∂/∂y: xdy Σ_z wz v_n,
+//   ∂/∂z: wxy Σ_z dwz v_n,
+// six products per node instead of twelve; the sums of grad_v up to
+// rounding.  With Q, also Σ N q_n of box field 3 (F-bar's ΔJ_n / m_n).
+template <typename T, int NV, bool Q>
+__device__ __forceinline__ T grad_v_rows(const BlockBox<3> &b, const W<3, T> &w, T (&gv)[3][3]) {
+    const int zb = w.base[2] - b.lo[2], xb = w.base[0] - b.lo[0], yb = w.base[1] - b.lo[1];
+    int az[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) az[j] = 16 * NV * ((zb + j) >> 2) + ((zb + j) & 3);
+    const int sy = 16 * NV * b.nz4 * b.nx4;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) gv[i][j] = T(0);
+    T q = T(0);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const T wxy = w.w[0][i] * w.w[1][k], dxy = w.dw[0][i] * w.w[1][k], xdy = w.w[0][i] * w.dw[1][k];
+            const int x = xb + i;
+            const int row = sy * (yb + k) + 16 * NV * b.nz4 * (x >> 2) + 4 * (x & 3);
+            T s[3] = {T(0), T(0), T(0)}, t[3] = {T(0), T(0), T(0)}, sq = T(0);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int li = row + az[j];
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const T vn = T(b.buf[li + 16 * c]);
+                    s[c] += w.w[2][j] * vn;
+                    t[c] += w.dw[2][j] * vn;
+                }
+                if (Q) sq += w.w[2][j] * T(b.buf[li + 48]);
+            }
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                gv[c][0] += dxy * s[c];
+                gv[c][1] += xdy * s[c];
+                gv[c][2] += wxy * t[c];
+            }
+            if (Q) q += wxy * sq;
+        }
+    }
+    return q;
+}
+
 // ΔF = I + dt ∇v, F_new = ΔF F (deform.jl:14-15)
 template <int D, typename T>
 __device__ __forceinline__ void deform_f(const PState<T> &P, long q, T dt, const T (&gv)[D][D], T (&dF)[D][D],
@@ -1550,8 +1669,13 @@ __device__ __forceinline__ void b_deform(Ctrl *c, const Geom &g, const PState<T>
     const bool staged = box.fits(D);
     if (staged) box.template stage<D>(g, G.v);
     T gv[D][D];
-    if (staged) grad_v<D, T>(g, w, LdsSrc<D, D>{&box}, 0, gv);
-    else grad_v<D, T>(g, w, GlobSrc{G.v}, 0, gv);
+    if constexpr (D == 3) {
+        if (staged) grad_v_rows<T, D, false>(box, w, gv);
+        else grad_v<D, T>(g, w, GlobSrc{G.v}, 0, gv);
+    } else {
+        if (staged) grad_v<D, T>(g, w, LdsSrc<D, D>{&box}, 0, gv);
+        else grad_v<D, T>(g, w, GlobSrc{G.v}, 0, gv);
+    }
     T dF[D][D], Fn[D][D];
     deform_f<D, T>(P, q, T(c->dt), gv, dF, Fn);
     const T J = det<D, T>(Fn);
@@ -1636,8 +1760,14 @@ __device__ __forceinline__ void b_fbar(Ctrl *c, const Geom &g, const PState<T> &
     }
     if (!live) return;
     T gv[D][D];
-    if (staged) grad_v<D, T>(g, w, LdsSrc<D, D + 1>{&box}, 0, gv);
-    else grad_v<D, T>(g, w, GlobSrc{G.v}, 0, gv);
+    T qsum = T(0);   // Σ N ΔJ_n/m_n from the staged box (3D)
+    if constexpr (D == 3) {
+        if (staged) qsum = grad_v_rows<T, D + 1, true>(box, w, gv);
+        else grad_v<D, T>(g, w, GlobSrc{G.v}, 0, gv);
+    } else {
+        if (staged) grad_v<D, T>(g, w, LdsSrc<D, D + 1>{&box}, 0, gv);
+        else grad_v<D, T>(g, w, GlobSrc{G.v}, 0, gv);
+    }
     T dF[D][D], Fn[D][D];
     deform_f<D, T>(P, p, T(c->dt), gv, dF, Fn);
     const T dJp = kDev<T> ? det1m<D, T>(dF) : det<D, T>(dF);
@@ -1652,11 +1782,14 @@ __device__ __forceinline__ void b_fbar(Ctrl *c, const Geom &g, const PState<T> &
     P.J[p] = J;
     if (B == 1) domain_update<D, T>(P, M, p, dF, Fn, J);
     T acc = T(0);
-    // FP64: Σ N ΔJ_n/ΔJ_p as volumetric.jl:27-32.  FP32: the nodal values are
+    // FP64: Σ N ΔJ_n/ΔJ_p as volumetric.jl:27-32 (3D: (Σ N ΔJ_n)/ΔJ_p, one
+    // division instead of 64).  FP32: the nodal values are
     // δJ_n = ΔJ_n - 1 and Σ N δJ_n is accumulated, so that with Σ N = 1
     // ΔJ̄/ΔJ_p - 1 = (Σ N δJ_n - δJ_p)/(1 + δJ_p) keeps its relative precision.
     const T rdiv = kDev<T> ? T(1) : dJp;
-    if (staged) {
+    if (D == 3 && staged) {
+        acc = kDev<T> ? qsum : qsum / rdiv;
+    } else if (staged) {
         gather_walk<D, T>(g, w, LdsSrc<D, D + 1>{&box}, [&](long, int li, T N, const T *, const int *) {
             acc += N * T(box.buf[li + 16 * D]) / rdiv;
         });

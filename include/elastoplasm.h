@@ -171,9 +171,11 @@ int ep_download_grid(ep_ctx *ctx, ep_grid_fields *g);
  * stays compact as the particles move.  every > 0: every `every` steps of
  * ep_step; every < 0 (the default, -20): adaptive, every -every steps the
  * host reads the device count of P2G blocks whose node box overflowed LDS
- * and re-sorts when they exceed 0.2 % of the blocks of those steps; 0:
- * never.  No counterpart in the reference (its particles keep their setup
- * order); results change only in the summation order of the scatters.
+ * and re-sorts when that rate exceeds both 0.2 % of the blocks of those
+ * steps and twice the rate of the first check after the previous sort (the
+ * floor a re-sort cannot lower); 0: never.  No counterpart in the
+ * reference (its particles keep their setup order); results change only in
+ * the summation order of the scatters.
  * Contexts in an ensemble are not re-sorted. */
 int ep_set_resort(ep_ctx *ctx, int32_t every);
 /* Launch shape (no counterpart in the reference): threads per particle /
