@@ -63,7 +63,7 @@ def test_ensemble_matches_single_contexts_and_oracle(trsfr, musl):
         d.close()
 
 
-def test_ensemble_fp32_tpic_usl_within_1e3():
+def test_ensemble_fp32_tpic_musl_within_1e3():
     lst, draw = _build(2 if EMU else 6, 2, "fp32")
     n = 3 if EMU else 40
     devs = _devices(lst)
@@ -76,6 +76,43 @@ def test_ensemble_fp32_tpic_usl_within_1e3():
         e = relerr(devs[i][0].mpts.s.tau, ref.mpts.s.tau)
         print(f"fp32 instance {i} (s={draw['s'][i]}, c={draw['c'][i]:.0f}): stress err {e:.2e}")
         assert e <= 1e-3
+    ens.close()
+    for _, d in devs:
+        d.close()
+
+
+def test_bench_mix_1000_steps():
+    """The 256-slump mix the benchmark runs (seed 43, nel 40x10 .. 320x80,
+    FP32, TPIC + mUSL), all 1000 steps: the smallest and the largest
+    instance against the FP64 oracle after 40 steps, then every instance
+    finite, inside its domain and with bounded dilation at step 1000."""
+    if EMU:
+        pytest.skip("1000 steps of 0.54M particles: GPU only")
+    lst, draw = ep.ic_ensemble(256, 43)
+    devs = _devices(lst)
+    ens = hip.Ensemble([d for _, d in devs])
+    ens.tune(5, 1e9)
+    r = ens.step(40, 1e9)
+    assert all(x[2] == 40 for x in r)
+    ens.download()
+    sizes = [ic.mpts.nmp for ic, _ in lst]
+    for i in (int(np.argmin(sizes)), int(np.argmax(sizes))):
+        ic, cfg = lst[i]
+        ref = clone(ic)
+        ref_cfg = clone(cfg)
+        ref_cfg.instr["ops"]["precision"] = 0
+        Oracle(ref.mesh, ref.mpts, ref.cmpr, ref_cfg.instr, threads=8).run(40, 1e9)
+        e = relerr(devs[i][0].mpts.s.tau, ref.mpts.s.tau)
+        print(f"instance {i} (nmp {sizes[i]}, s={draw['s'][i]}): fp32 stress err vs fp64 oracle at step 40 {e:.2e}")
+        assert e <= 1e-3
+    r = ens.step(960, 1e9)
+    assert all(x[2] == 960 for x in r)
+    ens.download()
+    for i, ((st, _), (ic, _)) in enumerate(zip(devs, lst)):
+        m = st.mpts
+        assert np.isfinite(m.s.tau).all() and np.isfinite(m.x).all(), i
+        assert ((m.x >= ic.mesh.xB[:, :1]) & (m.x <= ic.mesh.xB[:, 1:])).all(), i
+        assert 0.1 < m.J.min() and m.J.max() < 30.0, (i, m.J.min(), m.J.max())
     ens.close()
     for _, d in devs:
         d.close()
