@@ -36,7 +36,7 @@ from elastoplasm import hip  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 SURR_TOL = 3e-5      # scaled residual a learned-return prediction must meet
-RESORT = -20         # device re-sort policy: adaptive, checked every 20 steps (ep_set_resort)
+RESORT = -5          # device re-sort policy: adaptive, checked every 5 steps (ep_set_resort)
 MAT = dict(rho0=2650.0, E=1.0e6, nu=0.3, c0=20.0e3, cr=20.0e3, phi0=math.radians(30.0), phir=math.radians(30.0),
            Hp=0.0)
 

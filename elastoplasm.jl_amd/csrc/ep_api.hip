@@ -47,8 +47,11 @@ const char *kKernelNames[K_COUNT] = {"begin", "zero_grid", "p2g", "euler", "g2p"
 // Default re-sort policy of a context: adaptive, checking the p2g box
 // overflow count every 20 steps and re-sorting when more than 0.2 % of the
 // blocks of those steps fell back to global atomics.
-constexpr int kResortAdaptive = -20;
+constexpr int kResortAdaptive = -5;
 constexpr double kSpillFrac = 0.002;
+// re-sort once the p2g node boxes have grown by this factor over their size
+// just after the previous sort
+constexpr double kBoxGrowth = 1.1;
 
 // LU with partial pivoting of Del (same algorithm the device solve expects).
 void factor_del(int n, const double *A, DelLU &f) {
@@ -110,6 +113,8 @@ struct ep_ctx {
     long since_sort = 0;
     int spill_seen = 0;                // ctrl->spill at the last check
     double spill_floor = 0.0;          // overflow rate (per block and step) just after the last sort
+    unsigned long long box_seen = 0;   // ctrl->boxsum at the last check
+    double box_floor = 0.0;            // box nodes per block and step just after the last sort
     bool sorted_fresh = true;          // the next check measures that floor
     long resorts = 0;
     bool in_ensemble = false;          // an ensemble holds this context's field pointers
@@ -981,6 +986,7 @@ extern "C" int ep_step(ep_ctx *c, int32_t nsteps, const ep_step_params *par, ep_
     h.record = par->record_dt;
     h.steps = 0;
     h.spill = 0;
+    h.boxsum = 0;
     h.err = 0;
     h.j2cap = 0;
     h.s_acc = 0;
@@ -999,25 +1005,34 @@ extern "C" int ep_step(ep_ctx *c, int32_t nsteps, const ep_step_params *par, ep_
     HIPCHK(hipMemcpy(c->ctrl, &h, sizeof(Ctrl), hipMemcpyHostToDevice));
     c->ev_used = 0;
     c->spill_seen = 0;
+    c->box_seen = 0;
     for (int k = 0; k < nsteps; k++) {
         const int every = c->resort_every;
         if (every != 0 && !c->in_ensemble && c->since_sort >= (every > 0 ? every : -every)) {
             bool go = every > 0;
             if (!go) {
-                // adaptive: re-sort once more than kSpillFrac of the p2g blocks
-                // of the window fell back to global atomics
+                // adaptive: re-sort once the p2g node boxes have grown by
+                // kBoxGrowth since the last sort (the particles left their
+                // blocks' bricks), or once more than kSpillFrac of the p2g
+                // blocks of the window fell back to global atomics
                 int spill = 0;
+                unsigned long long box = 0;
                 HIPCHK(hipMemcpyAsync(&spill, &c->ctrl->spill, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+                HIPCHK(hipMemcpyAsync(&box, &c->ctrl->boxsum, sizeof(box), hipMemcpyDeviceToHost, c->stream));
                 HIPCHK(hipStreamSynchronize(c->stream));
                 const long nblk = (c->nslot + c->tb - 1) / c->tb;
-                const double rate = double(spill - c->spill_seen) / (double(nblk) * double(c->since_sort));
+                const double win = double(nblk) * double(c->since_sort);
+                const double rate = double(spill - c->spill_seen) / win;
+                const double brate = double(box - c->box_seen) / win;
                 c->spill_seen = spill;
+                c->box_seen = box;
                 if (c->sorted_fresh) {
-                    // the overflow a fresh sort leaves is the floor a re-sort cannot lower
+                    // what a fresh sort leaves is the floor a re-sort cannot lower
                     c->spill_floor = rate;
+                    c->box_floor = brate;
                     c->sorted_fresh = false;
                 } else {
-                    go = rate > kSpillFrac && rate > 2.0 * c->spill_floor;
+                    go = (rate > kSpillFrac && rate > 2.0 * c->spill_floor) || brate > kBoxGrowth * c->box_floor;
                 }
             }
             if (go) {

@@ -186,6 +186,18 @@ inline v16f emu_mfma_32x32x8bf16(v4s a, v4s b, v16f c) {
     __syncthreads();
     return d;
 }
+// v_mfma_f32_32x32x16_bf16: lane l holds A[i = l%32][k = 8(l/32)+v] and
+// B[k = 8(l/32)+v][j = l%32] (v = 0..7); D as for the 32x32x8 form.  The
+// same sum as two 32x32x8 products over k = 8(l/32)+{0..3} and +{4..7}.
+typedef short v8s __attribute__((vector_size(16)));
+inline v16f emu_mfma_32x32x16bf16(v8s a, v8s b, v16f c) {
+    v4s alo, ahi, blo, bhi;
+    std::memcpy(&alo, &a, 8);
+    std::memcpy(&ahi, reinterpret_cast<const char *>(&a) + 8, 8);
+    std::memcpy(&blo, &b, 8);
+    std::memcpy(&bhi, reinterpret_cast<const char *>(&b) + 8, 8);
+    return emu_mfma_32x32x8bf16(ahi, bhi, emu_mfma_32x32x8bf16(alo, blo, c));
+}
 
 // ---- runtime ----
 typedef int hipError_t;

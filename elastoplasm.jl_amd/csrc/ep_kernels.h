@@ -69,6 +69,7 @@ struct Ctrl {
     unsigned long long *vmax_part;  // [3][nblk] per-block maxima of |v_p| written by G2P
     int nblk;
     int spill;                      // particle blocks of p2g whose node box did not fit LDS (cumulative)
+    unsigned long long boxsum;      // node-box volumes of the p2g blocks (cumulative): how far the order has decayed
     // learned return mapping (FP32 + surrogate path)
     int qcount;                     // plastic trial states queued this step
     unsigned long long s_acc, s_fb; // predictions accepted / replaced by the FP64 return
@@ -296,6 +297,7 @@ template <int D>
 struct BlockBox {
     int lo[D], ext[D];
     int nloc;
+    long vol;                  // nodes of the block's box whether or not it fits
     int nz4, nx4, ey;          // box extent in groups of 4 along z and x; y extent (1 in 2D)
     int cap;                   // doubles of LDS behind buf
     bool any;                  // the block has a live particle
@@ -318,6 +320,9 @@ struct BlockBox {
 #pragma unroll
         for (int i = 0; i < D; i++) { mn[i] = uniform(sint[i]); mx[i] = uniform(sint[D + i]); }
         bool ok = mn[0] != 0x7fffffff;
+        vol = 1;
+#pragma unroll
+        for (int i = 0; i < D; i++) vol *= ok ? long(mx[i]) - long(mn[i]) + 4 : 0;
 #pragma unroll
         for (int i = 0; i < D; i++) {
             lo[i] = mn[i];
@@ -832,7 +837,10 @@ __device__ __forceinline__ void b_p2g(const Ctrl *c, const Geom &g, const PState
     BlockBox<D> box;
     box.begin((EP_LDS double *)smem, sint, live, w.base, ACC_LDS_P2G);
     const bool lds = box.fits(NV);
-    if (!lds && threadIdx.x == 0 && box.any) atomicAdd(const_cast<int *>(&c->spill), 1);
+    if (threadIdx.x == 0 && box.any) {
+        if (!lds) atomicAdd(const_cast<int *>(&c->spill), 1);
+        atomicAdd(const_cast<unsigned long long *>(&c->boxsum), (unsigned long long)box.vol);
+    }
     double *dst[NV];
     dst[0] = G.m;
 #pragma unroll
@@ -1808,7 +1816,10 @@ __device__ __forceinline__ void b_fbar(Ctrl *c, const Geom &g, const PState<T> &
 #pragma unroll
             for (int j = 0; j < D; j++) dF[i][j] = dF[i][j] + fm1 * ((i == j ? T(1) : T(0)) + dF[i][j]);
     } else {
-        const T f = pow(acc, T(1.0 / D));
+        // (ΔJ̄/ΔJ_p)^(1/d): cbrt / sqrt instead of pow (within an ulp of
+        // pow(acc, 1/d), at a fraction of its cost); a negative ratio is NaN
+        // as for pow, and ends the step as non-finite
+        const T f = !(acc >= T(0)) ? T(NAN) : (D == 3 ? cbrt(acc) : sqrt(acc));
 #pragma unroll
         for (int i = 0; i < D; i++)
 #pragma unroll

@@ -4,8 +4,10 @@
 // log strains and the material parameters (E, ν, c, φ, ψ) to the returned
 // principal Kirchhoff stresses and the plastic multiplier of the closed-form
 // Drucker-Prager return (DP.jl:1-62, 3D formulas).  Weights and activations
-// enter v_mfma_f32_32x32x8_bf16 as bf16, every sum accumulates in FP32, the
-// FP32 master weights are trained with Adam by the kernels below.
+// enter gfx950's v_mfma_f32_32x32x16_bf16 (the 8-input layer and the
+// 4-output transpose, K = 8, use v_mfma_f32_32x32x8_bf16) as bf16, every sum
+// accumulates in FP32, the FP32 master weights are trained with Adam by the
+// kernels below.
 //
 // One wave evaluates 32 samples: the samples are the N = 32 columns of every
 // MFMA, the features are rows, so a layer's 32x32 output tiles (FP32, in the
@@ -25,16 +27,30 @@ constexpr int NI = 8, NH = 128, NO = 4;
 constexpr int OW1 = 0, OB1 = OW1 + NH * NI, OW2 = OB1 + NH, OB2 = OW2 + NH * NH, OW3 = OB2 + NH,
               OB3 = OW3 + NH * NH, OW4 = OB3 + NH, OB4 = OW4 + NO * NH, NPARAM = OB4 + NO;
 
-// Position of A[i][k] in a swizzled operand image with KC chunks of 8 k:
-// tile (i/32, k/8), lane (i%32) + 32((k%8)/4), slot k%4.
+// Position of A[i][k] in a swizzled operand image with KC chunks of 8 k
+// (32x32x8 operands, 4 bf16 per lane): tile (i/32, k/8), lane
+// (i%32) + 32((k%8)/4), slot k%4.
 __host__ __device__ constexpr int swz(int i, int k, int KC) {
     return ((((i >> 5) * KC + (k >> 3)) * 64 + (i & 31) + 32 * ((k & 7) >> 2)) << 2) + (k & 3);
 }
-// forward images: W1 (4 row tiles x 1 chunk), W2, W3 (4 x 16), W4 (rows 0..3 of 1 x 16)
-constexpr int IF_W1 = 0, IF_W2 = IF_W1 + 4 * 1 * 256, IF_W3 = IF_W2 + 4 * 16 * 256, IF_W4 = IF_W3 + 4 * 16 * 256,
-              IF_N = IF_W4 + 1 * 16 * 256;
-// backward images (transposes): W4' (4 x 1, k 0..3), W3', W2' (4 x 16)
-constexpr int IB_W4T = 0, IB_W3T = IB_W4T + 4 * 1 * 256, IB_W2T = IB_W3T + 4 * 16 * 256, IB_N = IB_W2T + 4 * 16 * 256;
+// The same for 32x32x16 operands (8 bf16 per lane, KC chunks of 16 k): a
+// 16-k chunk is the two 8-k chunks 2c, 2c+1 side by side in each lane, slots
+// 0..3 from the first and 4..7 from the second: tile (i/32, k/16), lane
+// (i%32) + 32((k%8)/4), slot 4((k%16)/8) + k%4.  A layer's D tiles then feed
+// the next layer as they are: B chunk c of a lane is D elements 8c..8c+7,
+// rows 16c + 4(lane/32) + {0..3} and 16c + 8 + 4(lane/32) + {0..3}, which is
+// this k order, so the 16-k MFMA sums the same 16 products as the two 8-k
+// MFMAs it replaces.
+__host__ __device__ constexpr int swz16(int i, int k, int KC) {
+    return ((((i >> 5) * KC + (k >> 4)) * 64 + (i & 31) + 32 * ((k & 7) >> 2)) << 3) + 4 * ((k & 15) >> 3) + (k & 3);
+}
+// forward images: W1 (4 row tiles x 1 chunk of 8, swz), W2, W3 (4 x 8
+// chunks of 16, swz16), W4 (rows 0..3 of 1 x 8 chunks of 16, swz16)
+constexpr int IF_W1 = 0, IF_W2 = IF_W1 + 4 * 1 * 256, IF_W3 = IF_W2 + 4 * 8 * 512, IF_W4 = IF_W3 + 4 * 8 * 512,
+              IF_N = IF_W4 + 1 * 8 * 512;
+// backward images (transposes): W4' (4 x 1 chunk of 8, k 0..3, swz), W3',
+// W2' (4 x 8 chunks of 16, swz16)
+constexpr int IB_W4T = 0, IB_W3T = IB_W4T + 4 * 1 * 256, IB_W2T = IB_W3T + 4 * 8 * 512, IB_N = IB_W2T + 4 * 8 * 512;
 
 // ---- input / output scaling ------------------------------------------------
 // x = [ε1/0.2, ε2/0.2, ε3/0.2 (sorted, descending), (log10 E - 6.5)/1.5,
@@ -109,8 +125,8 @@ __device__ inline void zero_acc(v16f (&acc)[MT]) {
         for (int v = 0; v < 16; v++) acc[m][v] = 0.f;
 }
 
-// acc[m] += Σ_kc A(m, kc) B(kc); chunks outermost so the MT accumulation
-// chains interleave.
+// acc[m] = Σ_kc A(m, kc) B(kc) over KC chunks of 8 k (32x32x8, swz
+// images); chunks outermost so the MT accumulation chains interleave.
 template <int MT, int KC>
 __device__ inline void gemm_tile(const EP_LDS unsigned short *img, const v4s (&b)[KC], v16f (&acc)[MT], int lane) {
     zero_acc<MT>(acc);
@@ -121,6 +137,25 @@ __device__ inline void gemm_tile(const EP_LDS unsigned short *img, const v4s (&b
             const v4s a = *(const EP_LDS v4s *)(img + (((m * KC + kc) * 64 + lane) << 2));
             acc[m] = EP_MFMA_32X32X8_BF16(a, b[kc], acc[m]);
         }
+}
+// The same over 2*KC16 chunks of 8 k with 32x32x16 MFMAs (swz16 images): B
+// chunks 2c and 2c+1 form the 16-k operand of chunk c, A is one 16-byte LDS
+// read per lane.
+template <int MT, int KC16>
+__device__ inline void gemm_tile16(const EP_LDS unsigned short *img, const v4s (&b)[2 * KC16], v16f (&acc)[MT],
+                                   int lane) {
+    zero_acc<MT>(acc);
+#pragma unroll
+    for (int kc = 0; kc < KC16; kc++) {
+        v8s bb;
+#pragma unroll
+        for (int q = 0; q < 4; q++) { bb[q] = b[2 * kc][q]; bb[4 + q] = b[2 * kc + 1][q]; }
+#pragma unroll
+        for (int m = 0; m < MT; m++) {
+            const v8s a = *(const EP_LDS v8s *)(img + (((m * KC16 + kc) * 64 + lane) << 3));
+            acc[m] = EP_MFMA_32X32X16_BF16(a, bb, acc[m]);
+        }
+    }
 }
 
 // bias + SiLU of a 128-row layer, packed as the next layer's 16 B chunks;
@@ -151,12 +186,12 @@ __device__ inline void forward(const EP_LDS unsigned short *wf, const float *prm
         gemm_tile<4, 1>(wf + IF_W1, b1, acc, lane);
     }
     activate(acc, prm + OB1, lane, h1, TRAIN ? d1 : nullptr);
-    gemm_tile<4, 16>(wf + IF_W2, h1, acc, lane);
+    gemm_tile16<4, 8>(wf + IF_W2, h1, acc, lane);
     activate(acc, prm + OB2, lane, h2, TRAIN ? d2 : nullptr);
-    gemm_tile<4, 16>(wf + IF_W3, h2, acc, lane);
+    gemm_tile16<4, 8>(wf + IF_W3, h2, acc, lane);
     activate(acc, prm + OB3, lane, h3, TRAIN ? d3 : nullptr);
     v16f o[1];
-    gemm_tile<1, 16>(wf + IF_W4, h3, o, lane);
+    gemm_tile16<1, 8>(wf + IF_W4, h3, o, lane);
 #pragma unroll
     for (int q = 0; q < 4; q++) y[q] = o[0][q] + prm[OB4 + q];   // rows 0..3 live in lanes 0..31, v = 0..3
 }
@@ -266,10 +301,10 @@ __global__ void __launch_bounds__(256) k_mlp_fb(const unsigned short *img_f, con
         };
         grad_act(d3);
         store_rows(tb.del + DEL_G3 * tb.batch, tb.batch, col, g, lane);
-        gemm_tile<4, 16>((const EP_LDS unsigned short *)sb + IB_W3T, g, acc, lane);
+        gemm_tile16<4, 8>((const EP_LDS unsigned short *)sb + IB_W3T, g, acc, lane);
         grad_act(d2);
         store_rows(tb.del + DEL_G2 * tb.batch, tb.batch, col, g, lane);
-        gemm_tile<4, 16>((const EP_LDS unsigned short *)sb + IB_W2T, g, acc, lane);
+        gemm_tile16<4, 8>((const EP_LDS unsigned short *)sb + IB_W2T, g, acc, lane);
         grad_act(d1);
         store_rows(tb.del + DEL_G1 * tb.batch, tb.batch, col, g, lane);
     }
@@ -309,23 +344,30 @@ __global__ void __launch_bounds__(64) k_mlp_dw(const unsigned short *act, const 
     const int i = 32 * T.mt + (lane & 31);   // A row of this lane
     const int j = 32 * T.nt + (lane & 31);   // B column of this lane
     const bool arow_ok = i < M, bcol_ok = j < N;
+    // the batch is the K dimension: 32x32x16 products, lane l covering
+    // batch entries 16j + 8(l/32) + {0..7} of each chunk j (16 bytes)
     const long kspan = batch / nsplit, k0 = long(split) * kspan;
-    const unsigned short *ap = del + (arow0 + (arow_ok ? i : 0)) * batch + k0 + 4 * (lane >> 5);
-    const unsigned short *bp = act + (brow0 + (bcol_ok ? j : 0)) * batch + k0 + 4 * (lane >> 5);
+    const unsigned short *ap = del + (arow0 + (arow_ok ? i : 0)) * batch + k0 + 8 * (lane >> 5);
+    const unsigned short *bp = act + (brow0 + (bcol_ok ? j : 0)) * batch + k0 + 8 * (lane >> 5);
     v16f acc;
 #pragma unroll
     for (int v = 0; v < 16; v++) acc[v] = 0.f;
     float bsum = 0.f;
-    const v4s zero = {0, 0, 0, 0};
-    for (long k = 0; k < kspan; k += 8) {
-        v4s a = *(const v4s *)(ap + k);
-        v4s b = *(const v4s *)(bp + k);
+    v8s zero;
+#pragma unroll
+    for (int q = 0; q < 8; q++) zero[q] = 0;
+    for (long k = 0; k < kspan; k += 16) {
+        v8s a = *(const v8s *)(ap + k);
+        v8s b = *(const v8s *)(bp + k);
         if (!arow_ok) a = zero;
         if (!bcol_ok) b = zero;
-        if (T.nt == 0)
-            bsum += bf16_f((unsigned short)a[0]) + bf16_f((unsigned short)a[1]) + bf16_f((unsigned short)a[2]) +
-                    bf16_f((unsigned short)a[3]);
-        acc = EP_MFMA_32X32X8_BF16(a, b, acc);
+        if (T.nt == 0) {
+            float s4 = 0.f;
+#pragma unroll
+            for (int q = 0; q < 8; q++) s4 += bf16_f((unsigned short)a[q]);
+            bsum += s4;
+        }
+        acc = EP_MFMA_32X32X16_BF16(a, b, acc);
     }
     float *out = part + long(split) * NPARAM;
 #pragma unroll
@@ -352,15 +394,15 @@ __device__ inline void pack_param(int k, float w, unsigned short *img_f, unsigne
         img_f[IF_W1 + swz(o, i, 1)] = h;
     } else if (k >= OW2 && k < OB2) {
         const int o = (k - OW2) / NH, i = (k - OW2) % NH;
-        img_f[IF_W2 + swz(o, i, 16)] = h;
-        img_b[IB_W2T + swz(i, o, 16)] = h;
+        img_f[IF_W2 + swz16(o, i, 8)] = h;
+        img_b[IB_W2T + swz16(i, o, 8)] = h;
     } else if (k >= OW3 && k < OB3) {
         const int o = (k - OW3) / NH, i = (k - OW3) % NH;
-        img_f[IF_W3 + swz(o, i, 16)] = h;
-        img_b[IB_W3T + swz(i, o, 16)] = h;
+        img_f[IF_W3 + swz16(o, i, 8)] = h;
+        img_b[IB_W3T + swz16(i, o, 8)] = h;
     } else if (k >= OW4 && k < OB4) {
         const int o = (k - OW4) / NH, i = (k - OW4) % NH;
-        img_f[IF_W4 + swz(o, i, 16)] = h;
+        img_f[IF_W4 + swz16(o, i, 8)] = h;
         img_b[IB_W4T + swz(i, o, 1)] = h;
     }
 }

@@ -8,6 +8,7 @@
 #define EP_LDS
 inline void lds_atomic_add(double *p, double v) { std::atomic_ref<double>(*p).fetch_add(v); }
 #define EP_MFMA_32X32X8_BF16(a, b, c) emu_mfma_32x32x8bf16(a, b, c)
+#define EP_MFMA_32X32X16_BF16(a, b, c) emu_mfma_32x32x16bf16(a, b, c)
 inline int uniform(int v) { return v; }
 #else
 #include <hip/hip_runtime.h>
@@ -23,7 +24,13 @@ __device__ __forceinline__ void lds_atomic_add(EP_LDS double *p, double v) {
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 // MFMA operand types: 4 bf16 (as raw 16-bit patterns) and 16 FP32 accumulators
 typedef short v4s __attribute__((ext_vector_type(4)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 #define EP_MFMA_32X32X8_BF16(a, b, c) __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0)
+// gfx950: K = 16 per instruction, 8 bf16 per lane and operand, twice the
+// work of the 32x32x8 form in the same issue slots
+#define EP_MFMA_32X32X16_BF16(a, b, c) \
+    __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(v8bf, (a)), __builtin_bit_cast(v8bf, (b)), c, 0, 0, 0)
 #endif

@@ -227,8 +227,9 @@ class Device:
 
     def set_resort(self, every: int):
         """Re-sort the particles into brick-major cell order every `every`
-        steps; every < 0 (the library's default, -20): adaptively, checking
-        the P2G box overflow count every -every steps; 0: never."""
+        steps; every < 0 (the library's default, -5): adaptively, checking
+        the growth of the P2G node boxes and their overflow count every
+        -every steps; 0: never."""
         _check(self.L.ep_set_resort(self.h, int(every)))
 
     def set_layout(self, block: int, brick):

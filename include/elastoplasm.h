@@ -169,11 +169,13 @@ int ep_download_grid(ep_ctx *ctx, ep_grid_fields *g);
 /* Keep the particles in brick-major cell order: the device re-sorts the
  * particles by the cell of their current position, so each block's node box
  * stays compact as the particles move.  every > 0: every `every` steps of
- * ep_step; every < 0 (the default, -20): adaptive, every -every steps the
- * host reads the device count of P2G blocks whose node box overflowed LDS
- * and re-sorts when that rate exceeds both 0.2 % of the blocks of those
- * steps and twice the rate of the first check after the previous sort (the
- * floor a re-sort cannot lower); 0: never.  No counterpart in the
+ * ep_step; every < 0 (the default, -5): adaptive, every -every steps the
+ * host reads two device counters of the P2G blocks, the summed volume of
+ * their node boxes and the number whose box overflowed LDS, and re-sorts
+ * when the box volume per block has grown by 10 % over its value at the
+ * first check after the previous sort, or when the overflow rate exceeds
+ * both 0.2 % of the blocks and twice its rate at that check (the floor a
+ * re-sort cannot lower); 0: never.  No counterpart in the
  * reference (its particles keep their setup order); results change only in
  * the summation order of the scatters.
  * Contexts in an ensemble are not re-sorted. */
