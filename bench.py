@@ -214,6 +214,8 @@ def main():
     ap.add_argument("--full-fill-steps", type=int, default=20,
                     help="steps of the 16.8M-particle full-fill line (0: skip)")
     ap.add_argument("--surr-tol", type=float, default=SURR_TOL, help="residual tolerance of the learned return")
+    ap.add_argument("--finetune-steps", type=int, default=2000,
+                    help="Adam steps of fine-tuning on harvested in-step trial states (0: the C3 network as trained)")
     ap.add_argument("--resort", type=int, default=RESORT,
                     help="re-sort policy (ep_set_resort): K > 0 every K steps, -K adaptive every K steps, 0 never")
     args = ap.parse_args()
@@ -347,6 +349,36 @@ def main():
                 "kernels_us": {k: v * 1000 for k, v in pk32.items()},
                 "dtype": "f32 particles, f64 grid and near-degenerate eigen-decompositions"}
         del st32
+    if net is not None and args.finetune_steps > 0:
+        # fine-tuning on the step's own plastic trial states, harvested
+        # (ep_surrogate_harvest) every 5 steps of the same slump under another
+        # random cohesion field: the network never sees the timed trajectory
+        t0 = time.perf_counter()
+        st_h = copy.deepcopy(ic)
+        ep.grf.set_cohesion_field(st_h, **dict(GRF, seed=GRF["seed"] + 1))
+        dev_h = device(st_h, "fp32_surrogate")
+        harvested = 0
+        for k in range(args.warmup + args.steps):
+            dev_h.step(1, 1e9)
+            if k % 5 == 4:
+                harvested += net.harvest(dev_h, append=harvested > 0)
+        sth = dev_h.stats()
+        dev_h.close()
+        del st_h
+        t_h = time.perf_counter() - t0
+        acc0 = (int(sth.predictions) - int(sth.fallbacks)) / max(int(sth.predictions), 1)
+        t0 = time.perf_counter()
+        lft = net.train(args.finetune_steps, min(65536, harvested // 32 * 32), 1e-3, 1e-5)
+        t_f = time.perf_counter() - t0
+        print(f"[bench] surrogate fine-tuning: {harvested} trial states harvested from {args.warmup + args.steps} "
+              f"steps (cohesion field seed {GRF['seed'] + 1}; acceptance there with the C3 network {acc0:.3f}) in "
+              f"{t_h:.2f}s, {args.finetune_steps} Adam steps in {t_f:.2f}s, loss {lft[:5].mean():.3e} -> "
+              f"{lft[-20:].mean():.3e}", file=sys.stderr, flush=True)
+        surr["finetune"] = {"harvest_field_seed": GRF["seed"] + 1, "harvest_steps": args.warmup + args.steps,
+                            "harvest_every": 5, "samples": harvested, "steps": args.finetune_steps,
+                            "lr": [1e-3, 1e-5], "harvest_s": t_h, "train_s": t_f,
+                            "loss_first": float(lft[:5].mean()), "loss_last": float(lft[-20:].mean()),
+                            "acceptance_before_on_harvest_run": acc0}
     if net is not None:
         vs, els, pks, sts, stt = timed_run("fp32_surrogate")
         finite["fp32_surrogate"] = bool(np.isfinite(sts.mpts.s.tau).all())
@@ -361,6 +393,7 @@ def main():
         surr.update({"value": vs, "ms_per_step": 1000.0 * els / args.steps, "speedup_vs_fp64": vs / value,
                      "rel_stress_err_vs_fp64": errs, "rel_displacement_err_vs_fp64": errxs,
                      "predictions": pred, "accepted": pred - int(stt.fallbacks),
+                     "acceptance": ((pred - int(stt.fallbacks)) / pred) if pred else None,
                      "fallback_fraction": (int(stt.fallbacks) / pred) if pred else None,
                      "max_scaled_residual": float(stt.max_residual), "tol": args.surr_tol,
                      "kernels_us": {k: v * 1000 for k, v in pks.items()}})

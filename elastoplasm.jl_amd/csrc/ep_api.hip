@@ -1358,6 +1358,54 @@ extern "C" int ep_surrogate_predict(ep_surrogate *s, const float *x, int64_t n, 
     return rc;
 }
 
+extern "C" int ep_surrogate_harvest(ep_surrogate *s, ep_ctx *c, int32_t append, int64_t *count) {
+    if (!s || !c) return fail(EP_E_ARG, "ep_surrogate_harvest: null argument");
+    if (count) *count = 0;
+    if (c->cfg.precision != EP_FP32_SURROGATE || c->dim != 3 || !c->mat.q.idx)
+        return fail(EP_E_CONFIG, "ep_surrogate_harvest: needs a 3D EP_FP32_SURROGATE context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    int q = 0;
+    HIPCHK(hipMemcpy(&q, &c->ctrl->qcount, sizeof(int), hipMemcpyDeviceToHost));
+    const long keep = append ? s->ns : 0;
+    const long cap = keep + std::max(q, 0);
+    float *X = nullptr, *Y = nullptr;
+    double *raw = nullptr;
+    int *cnt = nullptr;
+    auto release = [&] { hipFree(X); hipFree(Y); hipFree(raw); hipFree(cnt); };
+    if (hipMalloc(&X, sizeof(float) * 8 * std::max(cap, 1L)) != hipSuccess ||
+        hipMalloc(&Y, sizeof(float) * 4 * std::max(cap, 1L)) != hipSuccess ||
+        hipMalloc(&raw, sizeof(double) * 6 * std::max(cap, 1L)) != hipSuccess ||
+        hipMalloc(&cnt, sizeof(int)) != hipSuccess) {
+        release();
+        return fail(EP_E_HIP, "hipMalloc failed");
+    }
+    int n = 0;
+    bool ok = hipMemset(cnt, 0, sizeof(int)) == hipSuccess;
+    if (ok && keep > 0)
+        ok = hipMemcpy(X, s->X, sizeof(float) * 8 * keep, hipMemcpyDeviceToDevice) == hipSuccess &&
+             hipMemcpy(Y, s->Y, sizeof(float) * 4 * keep, hipMemcpyDeviceToDevice) == hipSuccess &&
+             hipMemcpy(raw, s->raw, sizeof(double) * 6 * keep, hipMemcpyDeviceToDevice) == hipSuccess;
+    if (ok && q > 0) {
+        EP_LAUNCH(mlp::k_mlp_harvest, (q + 255) / 256, 256, c->stream, c->ctrl, c->psf, c->mat, keep, cap, X, Y, raw,
+                  cnt);
+        ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(c->stream) == hipSuccess &&
+             hipMemcpy(&n, cnt, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess;
+    }
+    if (!ok) {
+        release();
+        return fail(EP_E_HIP, "ep_surrogate_harvest: device copy or kernel failed");
+    }
+    hipFree(cnt);
+    hipFree(s->X); hipFree(s->Y); hipFree(s->raw);
+    s->X = X;
+    s->Y = Y;
+    s->raw = raw;
+    s->ns = keep + std::min<long>(n, cap - keep);
+    if (count) *count = n;
+    return EP_OK;
+}
+
 extern "C" int ep_attach_surrogate(ep_ctx *c, ep_surrogate *s, double tol) {
     if (!c) return fail(EP_E_ARG, "null context");
     if (c->cfg.precision != EP_FP32_SURROGATE) return fail(EP_E_CONFIG, "the context's precision is not EP_FP32_SURROGATE");

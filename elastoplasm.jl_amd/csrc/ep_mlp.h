@@ -467,6 +467,51 @@ __host__ __device__ inline double u01(unsigned long long seed, unsigned long lon
 // of the sequence seed + a·0x632BE59BD9B4E019, a = 0, 1, ...: the network
 // only ever stands in for regular shear returns.
 // raw[i] = [τ'1, τ'2, τ'3, Δλ, fs, S] in FP64.
+// The FP64 return of the isotropic trial stress Del ε (principal, sorted
+// strains e) with cohesion c (no softening within the return) and whether it
+// is a regular shear return onto the cone: trial tr, returned s, Δλ, the
+// trial overstress fs and the residual scale S.
+__device__ inline bool dp_sample(const double (&e)[3], double E, double nu, double c, double phi, double psi,
+                                 double (&tr)[3], double (&s)[6], double &G, double &dl, double &fs, double &S) {
+    const double K = E / (3.0 * (1.0 - 2.0 * nu));
+    G = E / (2.0 * (1.0 + nu));
+    const double D11 = K + 4.0 / 3.0 * G, D12 = K - 2.0 / 3.0 * G;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const double a0 = k == 0 ? D11 : D12, a1 = k == 1 ? D11 : D12, a2 = k == 2 ? D11 : D12;
+        tr[k] = a0 * e[0] + a1 * e[1] + a2 * e[2];
+        s[k] = tr[k];
+    }
+    s[3] = s[4] = s[5] = 0.0;
+    double e1 = 0.0;
+    dl = 0.0;
+    const int br = dp_return<6, double>(s, c, c, phi, 0.0, K, G, 0.0, e1, dl, psi);
+    double eta, etaB, xi;
+    dp_params<double>(phi, psi, c, eta, etaB, xi);
+    const double P = (tr[0] + tr[1] + tr[2]) / 3.0;
+    const double d0 = tr[0] - P, d1 = tr[1] - P, d2 = tr[2] - P;
+    fs = sqrt(0.5 * (d0 * d0 + d1 * d1 + d2 * d2)) + eta * P - xi;
+    S = sqrt(tr[0] * tr[0] + tr[1] * tr[1] + tr[2] * tr[2]) + xi;
+    return br == 1 && fs > 0.0 && xi - eta * ((s[0] + s[1] + s[2]) / 3.0) >= 0.0;
+}
+
+// sample i of a training set: features, scaled targets, raw FP64 values
+__device__ inline void put_sample(long i, const double (&e)[3], double E, double nu, double c, double phi, double psi,
+                                  bool good, const double (&tr)[3], const double (&s)[6], double G, double dl,
+                                  double fs, double S, float *X, float *Y, double *raw) {
+    double x[8];
+    features<double>(e, E, nu, c, phi, psi, x);
+#pragma unroll
+    for (int k = 0; k < 8; k++) X[i * 8 + k] = float(x[k]);
+#pragma unroll
+    for (int k = 0; k < 3; k++) Y[i * 4 + k] = good ? float((s[k] - tr[k]) / fs) : 0.f;
+    Y[i * 4 + 3] = good ? float(G * dl / fs) : 0.f;
+    if (raw) {
+        double *r = raw + i * 6;
+        r[0] = s[0]; r[1] = s[1]; r[2] = s[2]; r[3] = dl; r[4] = good ? fs : 0.0; r[5] = S;
+    }
+}
+
 __global__ void k_mlp_samples(long n, unsigned long long seed, float *X, float *Y, double *raw) {
     const long i = blockIdx.x * long(blockDim.x) + threadIdx.x;
     if (i >= n) return;
@@ -485,38 +530,38 @@ __global__ void k_mlp_samples(long n, unsigned long long seed, float *X, float *
         c = pow(10.0, 3.0 + 2.0 * u01(sd, i, 5));
         phi = (15.0 + 30.0 * u01(sd, i, 6)) * d2r;
         psi = phi * u01(sd, i, 7);
-        const double K = E / (3.0 * (1.0 - 2.0 * nu));
-        G = E / (2.0 * (1.0 + nu));
-        const double D11 = K + 4.0 / 3.0 * G, D12 = K - 2.0 / 3.0 * G;
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            const double a0 = k == 0 ? D11 : D12, a1 = k == 1 ? D11 : D12, a2 = k == 2 ? D11 : D12;
-            tr[k] = a0 * e[0] + a1 * e[1] + a2 * e[2];
-            s[k] = tr[k];
-        }
-        s[3] = s[4] = s[5] = 0.0;
-        double e1 = 0.0;
-        dl = 0.0;
-        const int br = dp_return<6, double>(s, c, c, phi, 0.0, K, G, 0.0, e1, dl, psi);
-        double eta, etaB, xi;
-        dp_params<double>(phi, psi, c, eta, etaB, xi);
-        const double P = (tr[0] + tr[1] + tr[2]) / 3.0;
-        const double d0 = tr[0] - P, d1 = tr[1] - P, d2 = tr[2] - P;
-        fs = sqrt(0.5 * (d0 * d0 + d1 * d1 + d2 * d2)) + eta * P - xi;
-        S = sqrt(tr[0] * tr[0] + tr[1] * tr[1] + tr[2] * tr[2]) + xi;
-        good = br == 1 && fs > 0.0 && xi - eta * ((s[0] + s[1] + s[2]) / 3.0) >= 0.0;
+        good = dp_sample(e, E, nu, c, phi, psi, tr, s, G, dl, fs, S);
     }
-    double x[8];
-    features<double>(e, E, nu, c, phi, psi, x);
+    put_sample(i, e, E, nu, c, phi, psi, good, tr, s, G, dl, fs, S, X, Y, raw);
+}
+
+// Training samples from the step's own trial states: the plastic trial
+// states k_fbar queued in the last step of an EP_FP32_SURROGATE context,
+// with the inputs k_surr gives the network (sorted principal log strains,
+// E, ν, the particle's cohesion and φ, ψ = 0) and the FP64 return of
+// k_mlp_samples as targets.  The cohesion is the one of the trial state,
+// c0 + Hp ϵpII before the step's return, max cr (ϵpII less Δλ/√3, the
+// increment of a shear return with ψ = 0).  States that do not end on the
+// cone by a shear return are left out; the others are appended at
+// first + atomicAdd(count) up to cap.
+__global__ void k_mlp_harvest(const Ctrl *c, PState<float> P, Mat M, long first, long cap, float *X, float *Y,
+                              double *raw, int *count) {
+    const long slot = blockIdx.x * long(blockDim.x) + threadIdx.x;
+    if (slot >= long(c->qcount)) return;
+    const long p = M.q.idx[slot];
+    double e[3];
 #pragma unroll
-    for (int k = 0; k < 8; k++) X[i * 8 + k] = float(x[k]);
-#pragma unroll
-    for (int k = 0; k < 3; k++) Y[i * 4 + k] = good ? float((s[k] - tr[k]) / fs) : 0.f;
-    Y[i * 4 + 3] = good ? float(G * dl / fs) : 0.f;
-    if (raw) {
-        double *r = raw + i * 6;
-        r[0] = s[0]; r[1] = s[1]; r[2] = s[2]; r[3] = dl; r[4] = good ? fs : 0.0; r[5] = S;
-    }
+    for (int k = 0; k < 3; k++) e[k] = double(M.q.e[k][slot]);
+    const double ep0 = double(P.epII[p]) - double(P.dlam[p]) * sqrt(1.0 / 3.0);
+    double cc = double(P.c0[p]) + M.Hp * ep0;
+    if (cc < double(P.cr[p])) cc = double(P.cr[p]);
+    // the network sees the FP32 cohesion k_surr forms
+    cc = double(float(cc));
+    const double E = double(float(M.E)), nu = double(float(M.nu)), phi = double(P.phi[p]);
+    double tr[3], s[6], G, dl, fs, S;
+    if (!dp_sample(e, E, nu, cc, phi, 0.0, tr, s, G, dl, fs, S)) return;
+    const long i = first + atomicAdd(count, 1);
+    if (i < cap) put_sample(i, e, E, nu, cc, phi, 0.0, true, tr, s, G, dl, fs, S, X, Y, raw);
 }
 
 // plain inference on [n][8] inputs (n a multiple of 32) -> [n][4]

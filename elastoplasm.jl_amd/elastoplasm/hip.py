@@ -77,7 +77,8 @@ EXPORTS = ["ep_version", "ep_last_error", "ep_create", "ep_destroy", "ep_upload_
            "ep_download_particles", "ep_download_grid", "ep_set_time", "ep_set_resort", "ep_set_layout", "ep_step", "ep_get_dts", "ep_profile",
            "ep_kernel_times", "ep_get_stats", "ep_surrogate_create", "ep_surrogate_destroy",
            "ep_surrogate_get_params", "ep_surrogate_set_params", "ep_surrogate_sample", "ep_surrogate_get_samples",
-           "ep_surrogate_train", "ep_surrogate_grad", "ep_surrogate_predict", "ep_attach_surrogate",
+           "ep_surrogate_train", "ep_surrogate_grad", "ep_surrogate_predict", "ep_surrogate_harvest",
+           "ep_attach_surrogate",
            "ep_ensemble_create", "ep_ensemble_destroy", "ep_ensemble_step", "ep_ensemble_tune",
            "ep_ensemble_get_stats", "ep_grf_gauss", "ep_grf_gauss_noalloc", "ep_slump_points"]
 
@@ -128,6 +129,7 @@ def load() -> C.CDLL:
     L.ep_surrogate_train.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_double, C.c_double, _fp]
     L.ep_surrogate_grad.argtypes = [C.c_void_p, _fp, _fp, C.c_int64, _fp, _fp]
     L.ep_surrogate_predict.argtypes = [C.c_void_p, _fp, C.c_int64, _fp]
+    L.ep_surrogate_harvest.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_int64)]
     L.ep_attach_surrogate.argtypes = [C.c_void_p, C.c_void_p, C.c_double]
     L.ep_ensemble_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]
     L.ep_ensemble_destroy.argtypes = [C.c_void_p]
@@ -340,6 +342,14 @@ class Surrogate:
         _check(self.L.ep_surrogate_get_samples(self.h, int(first), int(n), x.ctypes.data_as(_fp),
                                                 y.ctypes.data_as(_fp), raw.ctypes.data_as(_dp)))
         return x, y, raw
+
+    def harvest(self, dev: "Device", append: bool = True) -> int:
+        """Add the plastic trial states of dev's last step, with their FP64
+        returns, to the training set (ep_surrogate_harvest); returns the
+        number of samples added."""
+        n = C.c_int64(0)
+        _check(self.L.ep_surrogate_harvest(self.h, dev.h, 1 if append else 0, C.byref(n)))
+        return int(n.value)
 
     def train(self, steps: int, batch: int, lr0: float, lr1: float) -> np.ndarray:
         loss = np.zeros(max(int(steps), 1), np.float32)

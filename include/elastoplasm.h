@@ -280,8 +280,18 @@ void ep_surrogate_destroy(ep_surrogate *s);
 int ep_surrogate_get_params(ep_surrogate *s, float *params);
 int ep_surrogate_set_params(ep_surrogate *s, const float *params);
 /* Device training set of n samples: inputs x [n][8], scaled targets y [n][4],
- * raw [n][6] = FP64 (τ'1, τ'2, τ'3, Δλ, S, branch). */
+ * raw [n][6] = FP64 (τ'1, τ'2, τ'3, Δλ, fs, S): the returned principal
+ * stresses, Δλ, the trial overstress and the residual scale. */
 int ep_surrogate_sample(ep_surrogate *s, int64_t n, uint64_t seed);
+/* Training samples from a step's own trial states: the plastic trial states
+ * that the last ep_step of `ctx` (3D, EP_FP32_SURROGATE) queued for the
+ * learned return, as the network's inputs (sorted principal log strains, E,
+ * ν, the particle's cohesion and φ, ψ = 0), with the FP64 closed-form return
+ * as targets; states that do not end on the cone by a shear return are left
+ * out.  append = 0 replaces the training set, 1 appends to it; *count = the
+ * samples added.  For fine-tuning a network on the states a simulation
+ * actually produces. */
+int ep_surrogate_harvest(ep_surrogate *s, ep_ctx *ctx, int32_t append, int64_t *count);
 int ep_surrogate_get_samples(ep_surrogate *s, int64_t first, int64_t n, float *x, float *y, double *raw);
 /* `steps` Adam steps over consecutive batches of the training set, learning
  * rate cosine from lr0 to lr1; loss[steps] (may be NULL) = batch MSE. */

@@ -218,3 +218,70 @@ def test_in_step_acceptance_trained_vs_untrained(trained):
     assert relerr(a.mpts.s.tau, b.mpts.s.tau) <= 1e-3
     if not EMU:
         assert sa.predictions > 0 and acc_a > acc_b
+
+
+OTHER = dict(C1, c0=150.0, cr=150.0)
+
+
+def _surr_run(nw, cmpr, nsteps, harvest_into=None):
+    """nsteps of the 3D slump on the learned return; optionally harvest each
+    step's trial states into another network's training set."""
+    ic, cfg = slump(3, C_pf=1.0, cmpr=cmpr)
+    cs = clone(cfg)
+    cs.instr["precision"] = "fp32_surrogate"
+    cs.instr["ops"]["precision"] = 2
+    d = clone(ic)
+    dev = hip.Device(d.mesh, d.mpts, d.cmpr, cs.instr)
+    dev.attach_surrogate(nw, TOL_STEP)
+    tot = 0
+    for _ in range(nsteps):
+        assert dev.step(1, 100.0).steps == 1
+        if harvest_into is not None:
+            tot += harvest_into.harvest(dev, append=tot > 0)
+    st = dev.stats()
+    dev.close()
+    return st, tot
+
+
+def test_harvest_targets_and_fine_tuning(trained):
+    """ep_surrogate_harvest turns the plastic trial states a step queued into
+    training samples: every target is the FP64 return of its inputs (the
+    oracle's return of the inputs read back from the features), and
+    fine-tuning on the states of one slump (100 Pa cohesion) raises the
+    in-step acceptance at the step tolerance on another (150 Pa)."""
+    net, _ = trained
+    if EMU:
+        pytest.skip("needs the trained network and tens of steps")
+    tune = hip.Surrogate(seed=0)
+    tune.params = net.params
+    _, tot = _surr_run(net, WEAK, 20, harvest_into=tune)
+    assert tot >= 1024
+    X, Y, R = tune.samples(0, tot)
+    x = X.astype(np.float64)
+    e = mlp.EPS_SCALE * x[:, :3]
+    E = 10.0 ** (1.5 * x[:, 3] + 6.5)
+    nu = 0.125 * x[:, 4] + 0.325
+    c = 10.0 ** (x[:, 5] + 4.0)
+    phi = np.radians(15.0 * x[:, 6] + 30.0)
+    K, G = E / (3.0 * (1.0 - 2.0 * nu)), E / (2.0 * (1.0 + nu))
+    D11, D12 = K + 4.0 / 3.0 * G, K - 2.0 / 3.0 * G
+    tr = np.stack([D11 * e[:, k] + D12 * (e.sum(axis=1) - e[:, k]) for k in range(3)], axis=-1)
+    tp, dl, br = mlp.dp_return_principal(tr, c, phi, np.zeros_like(phi), K, G)
+    S = R[:, 5]
+    err = np.abs(tp - R[:, :3]).max(axis=1) / S
+    print(f"{tot} harvested states: shear returns {np.mean(br == 1):.4f}, target vs oracle max {err.max():.2e} "
+          f"median {np.median(err):.2e} (of S), features within [{x.min():.3f}, {x.max():.3f}]")
+    # the FP32 features carry the inputs to ~1e-7: the returns agree to that
+    assert np.mean(br == 1) >= 0.999 and err.max() <= 1e-5
+    assert np.allclose(Y[:, :3] * R[:, 4:5] + tr, R[:, :3], rtol=0, atol=1e-5 * S.max())
+    batch = min(65536, tot // 32 * 32)
+    loss = tune.train(400, batch, 1e-3, 1e-5)
+    assert loss[-20:].mean() < loss[:5].mean()
+    before, _ = _surr_run(net, OTHER, 20)
+    after, _ = _surr_run(tune, OTHER, 20)
+    tune.close()
+    a0 = (before.predictions - before.fallbacks) / max(before.predictions, 1)
+    a1 = (after.predictions - after.fallbacks) / max(after.predictions, 1)
+    print(f"in-step acceptance at tol {TOL_STEP} on the 150 Pa slump: C3-trained {a0:.3f} of "
+          f"{before.predictions}, fine-tuned on the 100 Pa slump's states {a1:.3f} of {after.predictions}")
+    assert after.predictions > 0 and a1 > a0
