@@ -362,15 +362,22 @@ struct BlockBox {
         if (ix0 < 0 || ix1 < 0 || ix2 < 0 || ix0 >= g.nno[0] || ix1 >= g.nno[1] || ix2 >= g.nno[2]) return -1;
         return long(ix2) + long(g.nno[2]) * (long(ix0) + long(g.nno[0]) * ix1);
     }
-    // LDS field v of every box node = src[v][node] (0 outside the grid); caller checks fits(NV)
-    template <int NV>
+    // the box memory as elements of type E (a gather of the FP32 path stages
+    // FP32 node values: the gathers convert every node value to the particle
+    // precision anyway, so staging them converted gives the same results with
+    // half the LDS reads)
+    template <typename E>
+    __device__ __forceinline__ EP_LDS E *as() const { return (EP_LDS E *)buf; }
+    // LDS field v of every box node = E(src[v][node]) (0 outside the grid); caller checks fits(NV)
+    template <int NV, typename E = double>
     __device__ __forceinline__ void stage(const Geom &g, const double *const *src) {
+        EP_LDS E *eb = as<E>();
         for (int k = threadIdx.x; k < nloc; k += blockDim.x) {
             int z, x, y;
             const long no = node(g, k, z, x, y);
             const int a = addr<NV>(z, x, y);
 #pragma unroll
-            for (int v = 0; v < NV; v++) buf[a + 16 * v] = no >= 0 ? src[v][no] : 0.0;
+            for (int v = 0; v < NV; v++) eb[a + 16 * v] = no >= 0 ? E(src[v][no]) : E(0);
         }
         __syncthreads();
     }
@@ -412,11 +419,11 @@ struct GlobSink {
         unsafeAtomicAdd(&dst[v][no], val);
     }
 };
-// Gather sources: the staged LDS box or the global nodal arrays.
-template <int D, int NV>
+// Gather sources: the staged LDS box (elements of type E) or the global nodal arrays.
+template <int D, int NV, typename E = double>
 struct LdsSrc {
     const BlockBox<D> *b;
-    __device__ __forceinline__ double operator()(int v, long, int li) const { return b->buf[li + 16 * v]; }
+    __device__ __forceinline__ double operator()(int v, long, int li) const { return double(b->template as<E>()[li + 16 * v]); }
 };
 struct GlobSrc {
     const double *const *src;
@@ -650,8 +657,8 @@ template <int D, typename T, typename Fn>
 __device__ __forceinline__ void scatter_walk(const Geom &g, const W<D, T> &w, const GlobSink &, Fn &&fn) {
     for_stencil_rot<D, T>(g, w, [&](long no, T N, const T *dN, const int *ix) { fn(no, 0, N, dN, ix); });
 }
-template <int D, typename T, int NV, typename Fn>
-__device__ __forceinline__ void gather_walk(const Geom &, const W<D, T> &w, const LdsSrc<D, NV> &s, Fn &&fn) {
+template <int D, typename T, int NV, typename E, typename Fn>
+__device__ __forceinline__ void gather_walk(const Geom &, const W<D, T> &w, const LdsSrc<D, NV, E> &s, Fn &&fn) {
     for_stencil_box<D, T, false, NV>(*s.b, w, fn);
 }
 template <int D, typename T, typename Fn>
@@ -872,10 +879,11 @@ __device__ __forceinline__ void b_p2g(const Ctrl *c, const Geom &g, const PState
     //   m N = (m wxy) wz,  p_i N = (p_i wxy) wz,
     //   f_x = (so_xx dxy + so_xy xdy) wz + (so_xz wxy) dwz, likewise y,
     //   f_z = (so_xz dxy + so_yz xdy + m g_z wxy) wz + (so_zz wxy) dwz,
-    // the same sums as the per-node form below up to rounding.  FP64 only:
-    // FP64 p2g 1138 -> 1105 us, while in FP32 the row factors raise the
-    // kernel from 89 to 132 VGPRs (5 -> 3 waves per SIMD, 949 -> 1116 us).
-    constexpr bool rows = D == 3 && !AFF && sizeof(T) == 8;
+    // the same sums as the per-node form below up to rounding.  FP64 p2g
+    // 1138 -> 1105 us; FP32 p2g 799 -> 754 us (once the box's min/max no
+    // longer cost a scalar loop; before, the extra VGPRs of the row factors
+    // had made FP32 slower, 949 -> 1116 us).
+    constexpr bool rows = D == 3 && !AFF;
     auto scatter_rows = [&](const LdsSink<D, NV> &sink) {
         if constexpr (rows) {
             T so[NS];
@@ -1029,7 +1037,7 @@ __device__ __forceinline__ void b_g2p(Ctrl *c, const Geom &g, const PState<T> &P
         const double *src[2 * D];
 #pragma unroll
         for (int d = 0; d < D; d++) { src[d] = G.v[d]; src[D + d] = G.a[d]; }
-        box.template stage<2 * D>(g, src);
+        box.template stage<2 * D, T>(g, src);
     }
     T vpic[D], aflip[D];
 #pragma unroll
@@ -1046,7 +1054,7 @@ __device__ __forceinline__ void b_g2p(Ctrl *c, const Geom &g, const PState<T> &P
             }
         });
     };
-    if (staged) gather(LdsSrc<D, 2 * D>{&box});
+    if (staged) gather(LdsSrc<D, 2 * D, T>{&box});
     else gather(GlobSrc{gsrc});
     const T dt = T(c->dt), C = T(M.C_pf);
     T vn[D];
@@ -1114,7 +1122,7 @@ __device__ __forceinline__ void b_bij(const Ctrl *c, const Geom &g, const PState
     const double *src[D];
 #pragma unroll
     for (int d = 0; d < D; d++) src[d] = G.v[d];
-    if (staged) box.template stage<D>(g, src);
+    if (staged) box.template stage<D, T>(g, src);
     T Bacc[D][D];
 #pragma unroll
     for (int i = 0; i < D; i++)
@@ -1133,7 +1141,7 @@ __device__ __forceinline__ void b_bij(const Ctrl *c, const Geom &g, const PState
             }
         });
     };
-    if (staged) gather(LdsSrc<D, D>{&box});
+    if (staged) gather(LdsSrc<D, D, T>{&box});
     else gather(GlobSrc{src});
     if (live) {
 #pragma unroll
@@ -1545,6 +1553,7 @@ __device__ __forceinline__ T grad_v_rows(const BlockBox<3> &b, const W<3, T> &w,
 #pragma unroll
         for (int j = 0; j < 3; j++) gv[i][j] = T(0);
     T q = T(0);
+    const EP_LDS T *bb = b.template as<T>();   // staged in the particle precision
 #pragma unroll
     for (int k = 0; k < 4; k++) {
 #pragma unroll
@@ -1558,11 +1567,11 @@ __device__ __forceinline__ T grad_v_rows(const BlockBox<3> &b, const W<3, T> &w,
                 const int li = row + az[j];
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
-                    const T vn = T(b.buf[li + 16 * c]);
+                    const T vn = bb[li + 16 * c];
                     s[c] += w.w[2][j] * vn;
                     t[c] += w.dw[2][j] * vn;
                 }
-                if (Q) sq += w.w[2][j] * T(b.buf[li + 48]);
+                if (Q) sq += w.w[2][j] * bb[li + 48];
             }
 #pragma unroll
             for (int c = 0; c < 3; c++) {
@@ -1675,13 +1684,13 @@ __device__ __forceinline__ void b_deform(Ctrl *c, const Geom &g, const PState<T>
     box.begin((EP_LDS double *)smem, sint, live, w.base, ACC_LDS_DEFORM);
     if (!live) w.retire(box.lo);
     const bool staged = box.fits(D);
-    if (staged) box.template stage<D>(g, G.v);
+    if (staged) box.template stage<D, T>(g, G.v);
     T gv[D][D];
     if constexpr (D == 3) {
         if (staged) grad_v_rows<T, D, false>(box, w, gv);
         else grad_v<D, T>(g, w, GlobSrc{G.v}, 0, gv);
     } else {
-        if (staged) grad_v<D, T>(g, w, LdsSrc<D, D>{&box}, 0, gv);
+        if (staged) grad_v<D, T>(g, w, LdsSrc<D, D, T>{&box}, 0, gv);
         else grad_v<D, T>(g, w, GlobSrc{G.v}, 0, gv);
     }
     T dF[D][D], Fn[D][D];
@@ -1760,9 +1769,10 @@ __device__ __forceinline__ void b_fbar(Ctrl *c, const Geom &g, const PState<T> &
                 const double m = G.m[no];
                 qv = m == 0.0 ? 0.0 : G.dJ[no] / m;
             }
+            EP_LDS T *eb = box.template as<T>();   // staged in the particle precision
 #pragma unroll
-            for (int d = 0; d < D; d++) box.buf[a + 16 * d] = no >= 0 ? G.v[d][no] : 0.0;
-            box.buf[a + 16 * D] = qv;
+            for (int d = 0; d < D; d++) eb[a + 16 * d] = no >= 0 ? T(G.v[d][no]) : T(0);
+            eb[a + 16 * D] = T(qv);
         }
         __syncthreads();
     }
@@ -1773,7 +1783,7 @@ __device__ __forceinline__ void b_fbar(Ctrl *c, const Geom &g, const PState<T> &
         if (staged) qsum = grad_v_rows<T, D + 1, true>(box, w, gv);
         else grad_v<D, T>(g, w, GlobSrc{G.v}, 0, gv);
     } else {
-        if (staged) grad_v<D, T>(g, w, LdsSrc<D, D + 1>{&box}, 0, gv);
+        if (staged) grad_v<D, T>(g, w, LdsSrc<D, D + 1, T>{&box}, 0, gv);
         else grad_v<D, T>(g, w, GlobSrc{G.v}, 0, gv);
     }
     T dF[D][D], Fn[D][D];
@@ -1798,8 +1808,8 @@ __device__ __forceinline__ void b_fbar(Ctrl *c, const Geom &g, const PState<T> &
     if (D == 3 && staged) {
         acc = kDev<T> ? qsum : qsum / rdiv;
     } else if (staged) {
-        gather_walk<D, T>(g, w, LdsSrc<D, D + 1>{&box}, [&](long, int li, T N, const T *, const int *) {
-            acc += N * T(box.buf[li + 16 * D]) / rdiv;
+        gather_walk<D, T>(g, w, LdsSrc<D, D + 1, T>{&box}, [&](long, int li, T N, const T *, const int *) {
+            acc += N * box.template as<T>()[li + 16 * D] / rdiv;
         });
     } else {
         for_stencil<D, T>(g, w, [&](long no, T N, const T *, const int *) {
