@@ -26,6 +26,8 @@ constexpr int NI = 8, NH = 128, NO = 4;
 // flat FP32 parameter vector: W[out][in] row-major, then the bias
 constexpr int OW1 = 0, OB1 = OW1 + NH * NI, OW2 = OB1 + NH, OB2 = OW2 + NH * NH, OW3 = OB2 + NH,
               OB3 = OW3 + NH * NH, OW4 = OB3 + NH, OB4 = OW4 + NO * NH, NPARAM = OB4 + NO;
+// the biases as the kernels keep them in LDS: b1, b2, b3, b4 side by side
+constexpr int BIAS_B1 = 0, BIAS_B2 = NH, BIAS_B3 = 2 * NH, BIAS_B4 = 3 * NH, BIAS_N = 3 * NH + NO;
 
 // Position of A[i][k] in a swizzled operand image with KC chunks of 8 k
 // (32x32x8 operands, 4 bf16 per lane): tile (i/32, k/8), lane
@@ -101,19 +103,19 @@ __host__ __device__ inline float bf16_f(unsigned short h) {
     __builtin_memcpy(&f, &u, 4);
     return f;
 }
+// four floats as bf16 (round to nearest even, the same rounding as bf16_rn)
 __device__ inline v4s pack4(float a, float b, float c, float d) {
     v4s r;
-    r[0] = short(bf16_rn(a));
-    r[1] = short(bf16_rn(b));
-    r[2] = short(bf16_rn(c));
-    r[3] = short(bf16_rn(d));
+    r[0] = short(ep_bf16_bits(a));
+    r[1] = short(ep_bf16_bits(b));
+    r[2] = short(ep_bf16_bits(c));
+    r[3] = short(ep_bf16_bits(d));
     return r;
 }
-__device__ inline float silu(float z) { return z / (1.f + expf(-z)); }
-__device__ inline float dsilu(float z) {
-    const float s = 1.f / (1.f + expf(-z));
-    return s * (1.f + z * (1.f - s));
-}
+// logistic σ(z) = 1 / (1 + e^-z) with the hardware exp2 and reciprocal (about
+// 1 ulp, far inside the bf16 rounding that follows); SiLU = z σ(z),
+// SiLU' = σ (1 + z (1 - σ))
+__device__ inline float sigmoid(float z) { return ep_rcp(1.f + ep_exp2(-1.4426950408889634f * z)); }
 // row of D element v in row tile mt
 __device__ inline int drow(int mt, int v, int lane) { return 32 * mt + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3); }
 
@@ -160,7 +162,7 @@ __device__ inline void gemm_tile16(const EP_LDS unsigned short *img, const v4s (
 
 // bias + SiLU of a 128-row layer, packed as the next layer's 16 B chunks;
 // optionally the bf16 SiLU' (for backward) and the activations themselves.
-__device__ inline void activate(const v16f (&acc)[4], const float *bias, int lane, v4s (&h)[16], v4s *dh) {
+__device__ inline void activate(const v16f (&acc)[4], const EP_LDS float *bias, int lane, v4s (&h)[16], v4s *dh) {
 #pragma unroll
     for (int m = 0; m < 4; m++)
 #pragma unroll
@@ -168,8 +170,13 @@ __device__ inline void activate(const v16f (&acc)[4], const float *bias, int lan
             float z[4];
 #pragma unroll
             for (int q = 0; q < 4; q++) z[q] = acc[m][4 * r + q] + bias[drow(m, 4 * r + q, lane)];
-            h[4 * m + r] = pack4(silu(z[0]), silu(z[1]), silu(z[2]), silu(z[3]));
-            if (dh) dh[4 * m + r] = pack4(dsilu(z[0]), dsilu(z[1]), dsilu(z[2]), dsilu(z[3]));
+            float sg[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) sg[q] = sigmoid(z[q]);
+            h[4 * m + r] = pack4(z[0] * sg[0], z[1] * sg[1], z[2] * sg[2], z[3] * sg[3]);
+            if (dh)
+                dh[4 * m + r] = pack4(sg[0] * (1.f + z[0] * (1.f - sg[0])), sg[1] * (1.f + z[1] * (1.f - sg[1])),
+                                      sg[2] * (1.f + z[2] * (1.f - sg[2])), sg[3] * (1.f + z[3] * (1.f - sg[3])));
         }
 }
 
@@ -178,22 +185,22 @@ __device__ inline void activate(const v16f (&acc)[4], const float *bias, int lan
 // the four outputs of their sample in y[0..3].  With TRAIN the bf16
 // activations and SiLU' are kept for the backward pass.
 template <bool TRAIN>
-__device__ inline void forward(const EP_LDS unsigned short *wf, const float *prm, v4s xb, int lane, float (&y)[4],
+__device__ inline void forward(const EP_LDS unsigned short *wf, const EP_LDS float *bias, v4s xb, int lane, float (&y)[4],
                                v4s (&h1)[16], v4s (&h2)[16], v4s (&h3)[16], v4s *d1, v4s *d2, v4s *d3) {
     v16f acc[4];
     {
         const v4s b1[1] = {xb};
         gemm_tile<4, 1>(wf + IF_W1, b1, acc, lane);
     }
-    activate(acc, prm + OB1, lane, h1, TRAIN ? d1 : nullptr);
+    activate(acc, bias + BIAS_B1, lane, h1, TRAIN ? d1 : nullptr);
     gemm_tile16<4, 8>(wf + IF_W2, h1, acc, lane);
-    activate(acc, prm + OB2, lane, h2, TRAIN ? d2 : nullptr);
+    activate(acc, bias + BIAS_B2, lane, h2, TRAIN ? d2 : nullptr);
     gemm_tile16<4, 8>(wf + IF_W3, h2, acc, lane);
-    activate(acc, prm + OB3, lane, h3, TRAIN ? d3 : nullptr);
+    activate(acc, bias + BIAS_B3, lane, h3, TRAIN ? d3 : nullptr);
     v16f o[1];
     gemm_tile16<1, 8>(wf + IF_W4, h3, o, lane);
 #pragma unroll
-    for (int q = 0; q < 4; q++) y[q] = o[0][q] + prm[OB4 + q];   // rows 0..3 live in lanes 0..31, v = 0..3
+    for (int q = 0; q < 4; q++) y[q] = o[0][q] + bias[BIAS_B4 + q];   // rows 0..3 live in lanes 0..31, v = 0..3
 }
 
 // copy a swizzled image (n elements, multiple of 8) from HBM to LDS
@@ -202,6 +209,13 @@ __device__ inline void load_image(EP_LDS unsigned short *dst, const unsigned sho
         const v4u v = *(const v4u *)(src + i);
         *(EP_LDS v4u *)(dst + i) = v;
     }
+}
+// the biases b1, b2, b3, b4 side by side in LDS (BIAS_*): each activation
+// reads 64 of them per lane, which from global memory put an L1 round trip
+// in every layer of every tile
+__device__ inline void load_bias(EP_LDS float *dst, const float *prm) {
+    for (int i = threadIdx.x; i < BIAS_N; i += blockDim.x)
+        dst[i] = i < BIAS_B4 ? prm[(i / NH == 0 ? OB1 : i / NH == 1 ? OB2 : OB3) + i % NH] : prm[OB4 + i - BIAS_B4];
 }
 
 // ---- training --------------------------------------------------------------
@@ -233,8 +247,10 @@ __global__ void __launch_bounds__(256) k_mlp_fb(const unsigned short *img_f, con
                                                 const float *prm, TrainBuf tb) {
     __shared__ __attribute__((aligned(16))) unsigned short sf[IF_N];
     __shared__ __attribute__((aligned(16))) unsigned short sb[IB_N];
+    __shared__ float sbias[BIAS_N];
     load_image((EP_LDS unsigned short *)sf, img_f, IF_N);
     load_image((EP_LDS unsigned short *)sb, img_b, IB_N);
+    load_bias((EP_LDS float *)sbias, prm);
     __syncthreads();
     const int lane = __lane_id(), nw = blockDim.x >> 6;
     const long ntile = tb.batch / 32;
@@ -251,7 +267,8 @@ __global__ void __launch_bounds__(256) k_mlp_fb(const unsigned short *img_f, con
         for (int q = 0; q < 4; q++) tb.act[(ACT_H0 + 4 * hh + q) * tb.batch + col] = (unsigned short)xb[q];
         v4s h1[16], h2[16], h3[16], d1[16], d2[16], d3[16];
         float y[4];
-        forward<true>((const EP_LDS unsigned short *)sf, prm, xb, lane, y, h1, h2, h3, d1, d2, d3);
+        forward<true>((const EP_LDS unsigned short *)sf, (const EP_LDS float *)sbias, xb, lane, y, h1, h2, h3, d1, d2,
+                      d3);
         store_rows(tb.act + ACT_H1 * tb.batch, tb.batch, col, h1, lane);
         store_rows(tb.act + ACT_H2 * tb.batch, tb.batch, col, h2, lane);
         store_rows(tb.act + ACT_H3 * tb.batch, tb.batch, col, h3, lane);
@@ -568,7 +585,9 @@ __global__ void k_mlp_harvest(const Ctrl *c, PState<float> P, Mat M, long first,
 __global__ void __launch_bounds__(256) k_mlp_predict(const unsigned short *img_f, const float *prm, const float *X,
                                                      long n, float *Y) {
     __shared__ __attribute__((aligned(16))) unsigned short sf[IF_N];
+    __shared__ float sbias[BIAS_N];
     load_image((EP_LDS unsigned short *)sf, img_f, IF_N);
+    load_bias((EP_LDS float *)sbias, prm);
     __syncthreads();
     const int lane = __lane_id(), nw = blockDim.x >> 6;
     for (long t = long(blockIdx.x) * nw + (threadIdx.x >> 6); t * 32 < n; t += long(gridDim.x) * nw) {
@@ -578,7 +597,8 @@ __global__ void __launch_bounds__(256) k_mlp_predict(const unsigned short *img_f
                              X[smp * 8 + 4 * hh + 3]);
         v4s h1[16], h2[16], h3[16];
         float y[4];
-        forward<false>((const EP_LDS unsigned short *)sf, prm, xb, lane, y, h1, h2, h3, nullptr, nullptr, nullptr);
+        forward<false>((const EP_LDS unsigned short *)sf, (const EP_LDS float *)sbias, xb, lane, y, h1, h2, h3, nullptr,
+                       nullptr, nullptr);
         if (hh == 0) {
 #pragma unroll
             for (int q = 0; q < 4; q++) Y[smp * 4 + q] = y[q];
@@ -605,12 +625,18 @@ struct SurrNet {
 // FP32 path.
 __global__ void __launch_bounds__(256) k_surr(Ctrl *c, PState<float> P, Mat M, SurrNet net) {
     __shared__ __attribute__((aligned(16))) unsigned short sf[IF_N];
+    __shared__ float sbias[BIAS_N];
     if (!c->active) return;
     load_image((EP_LDS unsigned short *)sf, net.img_f, IF_N);
+    load_bias((EP_LDS float *)sbias, net.prm);
     __syncthreads();
     const int count = c->qcount;
     const int lane = __lane_id(), nw = blockDim.x >> 6, hh = lane >> 5;
     const float E = float(M.E), nu = float(M.nu), Kf = float(M.Kc), Gf = float(M.Gc), Hp = float(M.Hp);
+    // statistics of the wave's tiles, added to the control block once at the
+    // end (atomics on the same three addresses after every tile serialise:
+    // 2.5 ms per step when most of the column yields)
+    unsigned long long n_acc = 0, n_fb = 0, rr_max = 0, any_bad = 0;
     for (long t = long(blockIdx.x) * nw + (threadIdx.x >> 6); t * 32 < count; t += long(gridDim.x) * nw) {
         const long slot = t * 32 + (lane & 31);
         const bool valid = slot < count;
@@ -628,7 +654,8 @@ __global__ void __launch_bounds__(256) k_surr(Ctrl *c, PState<float> P, Mat M, S
         const v4s xb = pack4(x[4 * hh], x[4 * hh + 1], x[4 * hh + 2], x[4 * hh + 3]);
         v4s h1[16], h2[16], h3[16];
         float y[4];
-        forward<false>((const EP_LDS unsigned short *)sf, net.prm, xb, lane, y, h1, h2, h3, nullptr, nullptr, nullptr);
+        forward<false>((const EP_LDS unsigned short *)sf, (const EP_LDS float *)sbias, xb, lane, y, h1, h2, h3, nullptr,
+                       nullptr, nullptr);
         bool ok = false, fb = false, bad = false;
         float res = 0.f;
         if (hh == 0 && valid) {
@@ -712,21 +739,24 @@ __global__ void __launch_bounds__(256) k_surr(Ctrl *c, PState<float> P, Mat M, S
             P.epII[p] = epn;
             P.dlam[p] = dl;
         }
-        const unsigned long long ma = __ballot(ok), mf = __ballot(fb), mb = __ballot(bad);
+        n_acc += __popcll(__ballot(ok));
+        n_fb += __popcll(__ballot(fb));
+        any_bad |= __ballot(bad);
         unsigned rb;
         __builtin_memcpy(&rb, &res, 4);
-        unsigned long long rr = rb;
+        rr_max = rb > rr_max ? rb : rr_max;   // non-negative floats order as their bits
+    }
+    unsigned long long rr = rr_max;
 #pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            const unsigned long long o = __shfl_xor(rr, off);
-            rr = o > rr ? o : rr;
-        }
-        if (lane == 0) {
-            if (ma) atomicAdd(&c->s_acc, (unsigned long long)__popcll(ma));
-            if (mf) atomicAdd(&c->s_fb, (unsigned long long)__popcll(mf));
-            atomicMax(&c->s_maxr, unsigned(rr));
-            if (mb) atomicOr(&c->err, ERR_NONFINITE);
-        }
+    for (int off = 32; off >= 1; off >>= 1) {
+        const unsigned long long o = __shfl_xor(rr, off);
+        rr = o > rr ? o : rr;
+    }
+    if (lane == 0) {
+        if (n_acc) atomicAdd(&c->s_acc, n_acc);
+        if (n_fb) atomicAdd(&c->s_fb, n_fb);
+        if (rr) atomicMax(&c->s_maxr, unsigned(rr));
+        if (any_bad) atomicOr(&c->err, ERR_NONFINITE);
     }
 }
 

@@ -9,6 +9,16 @@
 inline void lds_atomic_add(double *p, double v) { std::atomic_ref<double>(*p).fetch_add(v); }
 #define EP_MFMA_32X32X8_BF16(a, b, c) emu_mfma_32x32x8bf16(a, b, c)
 #define EP_MFMA_32X32X16_BF16(a, b, c) emu_mfma_32x32x16bf16(a, b, c)
+// bf16 bits of f rounded to nearest even (quiet NaN for NaN), 2^x, 1/x
+inline unsigned short ep_bf16_bits(float f) {
+    unsigned u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+inline float ep_exp2(float x) { return std::exp2(x); }
+inline float ep_rcp(float x) { return 1.0f / x; }
 inline int uniform(int v) { return v; }
 #else
 #include <hip/hip_runtime.h>
@@ -33,4 +43,9 @@ typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 // work of the 32x32x8 form in the same issue slots
 #define EP_MFMA_32X32X16_BF16(a, b, c) \
     __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(v8bf, (a)), __builtin_bit_cast(v8bf, (b)), c, 0, 0, 0)
+// bf16 bits of f rounded to nearest even: gfx950's v_cvt_pk_bf16_f32
+__device__ __forceinline__ unsigned short ep_bf16_bits(float f) { return __builtin_bit_cast(unsigned short, (__bf16)f); }
+// 2^x and 1/x by the transcendental unit (v_exp_f32, v_rcp_f32: about 1 ulp)
+__device__ __forceinline__ float ep_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ float ep_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 #endif
