@@ -216,6 +216,8 @@ def main():
     ap.add_argument("--surr-tol", type=float, default=SURR_TOL, help="residual tolerance of the learned return")
     ap.add_argument("--finetune-steps", type=int, default=2000,
                     help="Adam steps of fine-tuning on harvested in-step trial states (0: the C3 network as trained)")
+    ap.add_argument("--breakdown-steps", type=int, default=10,
+                    help="steps after the timed ones that time every kernel (per-kernel breakdown)")
     ap.add_argument("--resort", type=int, default=RESORT,
                     help="re-sort policy (ep_set_resort): K > 0 every K steps, -K adaptive every K steps, 0 never")
     args = ap.parse_args()
@@ -259,23 +261,49 @@ def main():
         return dev
 
     def timed_run(precision):
-        """warmup + timed steps from the initial state; returns (pu/s, s,
-        per-kernel {name: ms per launch}, final state, stats)"""
+        """warmup + timed steps from the initial state, then `args.breakdown_steps`
+        more.  The warmup times every kernel with HIP events to find the
+        dominant one; the timed steps time only that kernel (its average launch
+        duration is the roofline's), so the events of the other kernels stay
+        out of the wall time; the steps after them time every kernel for the
+        per-kernel breakdown.  Returns (pu/s, s, dominant kernel, its ms per
+        launch, {kernel: ms per launch}, {kernel: ms per step}, state, stats)."""
         st = copy.deepcopy(ic)
         dev = device(st, precision)
+        kbytes = KERNEL_BYTES_3D["fp64" if precision == "fp64" else "fp32"]
         if args.warmup:
+            dev.profile(True)
             dev.step(args.warmup, 1e9)
-        dev.profile(True)
+            kw = dev.kernel_times()
+            dom = max((k for k in kw if k in kbytes), key=lambda k: kw[k][0] / kw[k][1])
+            dev.profile_kernel(dom)
+        else:
+            dom = None
+            dev.profile(True)
         t0 = time.perf_counter()
         r = dev.step(args.steps, 1e9)
         el = time.perf_counter() - t0
         kt = dev.kernel_times()
-        dev.profile(False)
         assert r.steps == args.steps, "steps were skipped"
+        if dom is None:
+            dom = max((k for k in kt if k in kbytes), key=lambda k: kt[k][0] / kt[k][1])
+        dom_ms = kt[dom][0] / kt[dom][1]
+        per_launch, per_step = {}, {}
+        if args.breakdown_steps:
+            dev.profile(True)
+            dev.step(args.breakdown_steps, 1e9)
+            kb = dev.kernel_times()
+            per_launch = {k: ms / n for k, (ms, n) in kb.items()}
+            per_step = {k: ms / args.breakdown_steps for k, (ms, n) in kb.items()}
+            top = max((k for k in per_launch if k in kbytes), key=lambda k: per_launch[k])
+            if top != dom:
+                print(f"[bench] {precision}: {top} ({per_launch[top]*1e3:.0f} us) outran {dom} "
+                      f"({per_launch[dom]*1e3:.0f} us) after the timed steps", file=sys.stderr, flush=True)
+        dev.profile(False)
         dev.download()
         stats = dev.stats()
         dev.close()
-        return nmp * args.steps / el, el, {k: ms / n for k, (ms, n) in kt.items()}, st, stats
+        return nmp * args.steps / el, el, dom, dom_ms, per_launch, per_step, st, stats
 
     def long_run(precision, steps, tail=100):
         tail = max(1, min(tail, steps // 2))
@@ -309,27 +337,25 @@ def main():
               f"{out['kernels_us_last_steps'].get('p2g', float('nan')):.0f} us", file=sys.stderr, flush=True)
         return out
 
-    value, el, per_kernel, st64, _ = timed_run("fp64")
+    value, el, dom, dom_ms, per_kernel, per_step, st64, _ = timed_run("fp64")
     ms_step = 1000.0 * el / args.steps
-    print(f"[bench] {args.steps} steps in {el:.3f}s: {ms_step:.3f} ms/step, {value:.4g} pu/s", file=sys.stderr,
-          flush=True)
+    print(f"[bench] {args.steps} steps in {el:.3f}s: {ms_step:.3f} ms/step, {value:.4g} pu/s; {dom} "
+          f"{dom_ms*1000:.1f} us/launch over the timed steps", file=sys.stderr, flush=True)
     for k, v in sorted(per_kernel.items(), key=lambda kv: -kv[1]):
         print(f"[bench]   {k:10s} {v*1000:9.1f} us/launch", file=sys.stderr)
-    kbytes = KERNEL_BYTES_3D["fp64"]
-    dom = max((k for k in per_kernel if k in kbytes), key=lambda k: per_kernel[k])
-    kb = kbytes[dom] * (1 if dim == 3 else 0.5)
-    achieved = kb * nmp / (per_kernel[dom] * 1e-3) / 1e9
-    step_ms = sum(per_kernel.values())
+    kb = KERNEL_BYTES_3D["fp64"][dom] * (1 if dim == 3 else 0.5)
+    achieved = kb * nmp / (dom_ms * 1e-3) / 1e9
+    step_ms = sum(per_step.values())
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload, "fp64", dom),
-            "kernel": dom, "kernel_ms": per_kernel[dom], "bytes_per_particle": kb,
+            "kernel": dom, "kernel_ms": dom_ms, "bytes_per_particle": kb,
             "algorithmic_bytes": kb * nmp}
     step_roof = {"bytes_per_particle": B_ALG[dim], "achieved": B_ALG[dim] * value / 1e9,
                  "frac": B_ALG[dim] * value / 1e9 / HBM_PEAK_GBS, "kernel_ms_per_step": step_ms}
     finite = {"fp64": bool(np.isfinite(st64.mpts.s.tau).all())}
     fp32 = None
     if args.precision in ("all", "fp32"):
-        v32, el32, pk32, st32, _ = timed_run("fp32")
+        v32, el32, d32, d32_ms, pk32, _, st32, _ = timed_run("fp32")
         finite["fp32"] = bool(np.isfinite(st32.mpts.s.tau).all())
         err32 = float(np.linalg.norm(st32.mpts.s.tau - st64.mpts.s.tau) / np.linalg.norm(st64.mpts.s.tau))
         x0 = ic.mpts.x
@@ -338,13 +364,12 @@ def main():
               file=sys.stderr, flush=True)
         for k, v in sorted(pk32.items(), key=lambda kv: -kv[1]):
             print(f"[bench]   fp32 {k:10s} {v*1000:9.1f} us/launch", file=sys.stderr)
-        d32 = max((k for k in pk32 if k in KERNEL_BYTES_3D["fp32"]), key=lambda k: pk32[k])
         kb32 = KERNEL_BYTES_3D["fp32"][d32] * (1 if dim == 3 else 0.5)
-        a32 = kb32 * nmp / (pk32[d32] * 1e-3) / 1e9
+        a32 = kb32 * nmp / (d32_ms * 1e-3) / 1e9
         fp32 = {"value": v32, "ms_per_step": 1000.0 * el32 / args.steps, "speedup_vs_fp64": v32 / value,
                 "roofline": {"bound": "hbm", "achieved": a32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": a32 / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload, "fp32", d32),
-                             "kernel": d32, "kernel_ms": pk32[d32], "bytes_per_particle": kb32},
+                             "kernel": d32, "kernel_ms": d32_ms, "bytes_per_particle": kb32},
                 "rel_stress_err_vs_fp64": err32, "rel_displacement_err_vs_fp64": errx32,
                 "kernels_us": {k: v * 1000 for k, v in pk32.items()},
                 "dtype": "f32 particles, f64 grid and near-degenerate eigen-decompositions"}
@@ -380,7 +405,7 @@ def main():
                             "loss_first": float(lft[:5].mean()), "loss_last": float(lft[-20:].mean()),
                             "acceptance_before_on_harvest_run": acc0}
     if net is not None:
-        vs, els, pks, sts, stt = timed_run("fp32_surrogate")
+        vs, els, _, _, pks, _, sts, stt = timed_run("fp32_surrogate")
         finite["fp32_surrogate"] = bool(np.isfinite(sts.mpts.s.tau).all())
         errs = float(np.linalg.norm(sts.mpts.s.tau - st64.mpts.s.tau) / np.linalg.norm(st64.mpts.s.tau))
         x0 = ic.mpts.x

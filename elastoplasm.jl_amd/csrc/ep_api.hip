@@ -44,9 +44,10 @@ enum KernelId { K_BEGIN, K_ZERO, K_P2G, K_EULER, K_G2P, K_BIJ, K_AUGV, K_DEFORM,
 const char *kKernelNames[K_COUNT] = {"begin", "zero_grid", "p2g", "euler", "g2p", "bij", "augv", "deform", "fbar",
                                      "surrogate", "end", "resort", "nonlocal"};
 
-// Default re-sort policy of a context: adaptive, checking the p2g box
-// overflow count every 20 steps and re-sorting when more than 0.2 % of the
-// blocks of those steps fell back to global atomics.
+// Default re-sort policy of a context: adaptive, checked every 5 steps;
+// re-sort when the p2g node boxes have grown by kBoxGrowth over their volume
+// after the last sort, or when more than kSpillFrac of the p2g blocks of the
+// window fell back to global atomics.
 constexpr int kResortAdaptive = -5;
 constexpr double kSpillFrac = 0.002;
 // re-sort once the p2g node boxes have grown by this factor over their size
@@ -147,6 +148,7 @@ struct ep_ctx {
     Mat mat{};
     // profiling
     bool profile = false;
+    int prof_kid = -1;                  // >= 0: only this kernel is timed
     std::vector<hipEvent_t> ev_pool;
     std::vector<int> ev_kernel;
     size_t ev_used = 0;
@@ -772,7 +774,7 @@ extern "C" int ep_download_grid(ep_ctx *c, ep_grid_fields *gf) {
 // ---------------------------------------------------------------------------
 template <typename L>
 static void timed(ep_ctx *c, int kid, L &&launch) {
-    if (c->profile) {
+    if (c->profile && (c->prof_kid < 0 || c->prof_kid == kid)) {
         if (c->ev_used + 2 > c->ev_pool.size()) {
             for (int k = 0; k < 64; k++) {
                 hipEvent_t e;
@@ -1090,6 +1092,19 @@ extern "C" int ep_get_dts(ep_ctx *c, double *dts, int64_t cap, int64_t *n) {
 extern "C" int ep_profile(ep_ctx *c, int32_t enable) {
     if (!c) return fail(EP_E_ARG, "null context");
     c->profile = enable != 0;
+    c->prof_kid = -1;
+    for (int k = 0; k < K_COUNT; k++) { c->k_ms[k] = 0.0; c->k_launches[k] = 0; }
+    return EP_OK;
+}
+
+extern "C" int ep_profile_kernel(ep_ctx *c, const char *name) {
+    if (!c || !name) return fail(EP_E_ARG, "null argument");
+    int kid = -1;
+    for (int k = 0; k < K_COUNT; k++)
+        if (std::strcmp(name, kKernelNames[k]) == 0) kid = k;
+    if (kid < 0) return fail(EP_E_ARG, std::string("ep_profile_kernel: no kernel named ") + name);
+    c->profile = true;
+    c->prof_kid = kid;
     for (int k = 0; k < K_COUNT; k++) { c->k_ms[k] = 0.0; c->k_launches[k] = 0; }
     return EP_OK;
 }
@@ -1099,7 +1114,7 @@ extern "C" int ep_kernel_times(ep_ctx *c, int32_t max_kernels, char *names, int3
     if (!c || !count) return fail(EP_E_ARG, "null argument");
     int k = 0;
     for (int i = 0; i < K_COUNT && k < max_kernels; i++) {
-        if (c->k_launches[i] == 0) continue;
+        if (c->k_launches[i] == 0 || (c->prof_kid >= 0 && i != c->prof_kid)) continue;
         if (names && name_len > 0) {
             std::strncpy(names + size_t(k) * name_len, kKernelNames[i], name_len - 1);
             names[size_t(k) * name_len + name_len - 1] = 0;

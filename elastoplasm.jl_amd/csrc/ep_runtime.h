@@ -20,6 +20,7 @@ inline unsigned short ep_bf16_bits(float f) {
 inline float ep_exp2(float x) { return std::exp2(x); }
 inline float ep_rcp(float x) { return 1.0f / x; }
 inline int uniform(int v) { return v; }
+#define EP_WAVES_PER_EU(n)
 #else
 #include <hip/hip_runtime.h>
 #define EP_BLOCK 256
@@ -29,6 +30,8 @@ inline int uniform(int v) { return v; }
 __device__ __forceinline__ void lds_atomic_add(EP_LDS double *p, double v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
+// register budget of a kernel: at least n waves per SIMD (at most 512/n VGPRs)
+#define EP_WAVES_PER_EU(n) __attribute__((amdgpu_waves_per_eu(n)))
 #define EP_LAUNCH(kern, grid, block, stream, ...) hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, stream, __VA_ARGS__)
 // a value every lane of the wave holds alike, moved to a scalar register
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }

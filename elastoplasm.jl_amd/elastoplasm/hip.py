@@ -74,7 +74,7 @@ class EpStats(C.Structure):
 
 
 EXPORTS = ["ep_version", "ep_last_error", "ep_create", "ep_destroy", "ep_upload_grid", "ep_upload_particles",
-           "ep_download_particles", "ep_download_grid", "ep_set_time", "ep_set_resort", "ep_set_layout", "ep_step", "ep_get_dts", "ep_profile",
+           "ep_download_particles", "ep_download_grid", "ep_set_time", "ep_set_resort", "ep_set_layout", "ep_step", "ep_get_dts", "ep_profile", "ep_profile_kernel",
            "ep_kernel_times", "ep_get_stats", "ep_surrogate_create", "ep_surrogate_destroy",
            "ep_surrogate_get_params", "ep_surrogate_set_params", "ep_surrogate_sample", "ep_surrogate_get_samples",
            "ep_surrogate_train", "ep_surrogate_grad", "ep_surrogate_predict", "ep_surrogate_harvest",
@@ -116,6 +116,7 @@ def load() -> C.CDLL:
     L.ep_step.argtypes = [C.c_void_p, C.c_int32, C.POINTER(EpStepParams), C.POINTER(EpStepResult)]
     L.ep_get_dts.argtypes = [C.c_void_p, _dp, C.c_int64, C.POINTER(C.c_int64)]
     L.ep_profile.argtypes = [C.c_void_p, C.c_int32]
+    L.ep_profile_kernel.argtypes = [C.c_void_p, C.c_char_p]
     L.ep_kernel_times.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_int32, _dp,
                                   C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
     L.ep_get_stats.argtypes = [C.c_void_p, C.POINTER(EpStats)]
@@ -268,6 +269,10 @@ class Device:
 
     def profile(self, on: bool = True):
         _check(self.L.ep_profile(self.h, 1 if on else 0))
+
+    def profile_kernel(self, name: str):
+        """Time only the launches of kernel `name` (a key of kernel_times)."""
+        _check(self.L.ep_profile_kernel(self.h, name.encode()))
 
     def kernel_times(self) -> dict:
         nmax, nl = 32, 32

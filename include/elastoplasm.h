@@ -194,8 +194,13 @@ int ep_step(ep_ctx *ctx, int32_t nsteps, const ep_step_params *par, ep_step_resu
 /* dt of each step of the last ep_step call with record_dt=1. */
 int ep_get_dts(ep_ctx *ctx, double *dts, int64_t cap, int64_t *n);
 
-/* Per-kernel HIP-event timing on the context's stream. */
+/* Per-kernel HIP-event timing on the context's stream (no counterpart in the
+ * reference).  ep_profile times every kernel launch; ep_profile_kernel times
+ * only the launches of the kernel of that name ("p2g", "fbar", ...; the names
+ * ep_kernel_times reports), so that the events of the other kernels stay out
+ * of a timed run.  Both reset the accumulated times. */
 int ep_profile(ep_ctx *ctx, int32_t enable);
+int ep_profile_kernel(ep_ctx *ctx, const char *kernel);
 int ep_kernel_times(ep_ctx *ctx, int32_t max_kernels, char *names, int32_t name_len,
                     double *total_ms, int64_t *launches, int32_t *count);
 int ep_get_stats(ep_ctx *ctx, ep_stats *st);
