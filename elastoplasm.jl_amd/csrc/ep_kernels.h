@@ -683,6 +683,29 @@ __device__ __forceinline__ unsigned long long block_max_bits(unsigned long long 
     return b;
 }
 
+// N block maxima with one pair of barriers (sred holds N x 16 entries, so
+// blocks of at most 1024 threads); thread 0 returns them in b
+template <int N>
+__device__ __forceinline__ void block_max_bits_n(unsigned long long (&b)[N], unsigned long long *sred) {
+#pragma unroll
+    for (int d = 0; d < N; d++)
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const unsigned long long o = __shfl_xor(b[d], off);
+            b[d] = o > b[d] ? o : b[d];
+        }
+    const int wv = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int d = 0; d < N; d++) sred[16 * d + wv] = b[d];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int d = 0; d < N; d++)
+            for (int k = 1; k < nw; k++) b[d] = sred[16 * d + k] > b[d] ? sred[16 * d + k] : b[d];
+    __syncthreads();
+}
+
 template <int D>
 __device__ __forceinline__ void stencil_base(const Geom &g, const double *xp, int *base) {
 #pragma unroll
@@ -735,7 +758,7 @@ __global__ void k_begin(Ctrl *c, Geom g, Mat M) {
 // End of step: vmax = max over G2P's per-block maxima (the atomic max of
 // n2p.jl:37), then time.t[1] += dt (worflow.jl:38, 84).
 __device__ __forceinline__ void b_end(Ctrl *c) {
-    __shared__ unsigned long long sred[16];
+    __shared__ unsigned long long sred[48];   // 3 x 16 waves of a 1024-thread block
     if (!c->active) return;
     const int nblk = c->nblk;
     const unsigned long long *part = c->vmax_part;
@@ -747,12 +770,10 @@ __device__ __forceinline__ void b_end(Ctrl *c) {
             b[d] = o > b[d] ? o : b[d];
         }
     }
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        const unsigned long long r = block_max_bits(b[d], sred);
-        if (threadIdx.x == 0) c->vmax[d] = r;
-    }
+    block_max_bits_n<3>(b, sred);
     if (threadIdx.x != 0) return;
+#pragma unroll
+    for (int d = 0; d < 3; d++) c->vmax[d] = b[d];
     c->t = c->t + c->dt;
     c->it += 1;
     c->steps += 1;
@@ -1022,7 +1043,7 @@ template <int D, int B, typename T>
 __device__ __forceinline__ void b_g2p(Ctrl *c, const Geom &g, const PState<T> &P, const GState &G, const Mat &M, long blk) {
     __shared__ double smem[ACC_LDS_G2P];
     __shared__ int sint[8];
-    __shared__ unsigned long long sred[16];
+    __shared__ unsigned long long sred[16 * D];
     if (!c->active) return;
     const long p = blk * long(blockDim.x) + threadIdx.x;
     const long q = p < P.n ? p : 0;
@@ -1063,6 +1084,7 @@ __device__ __forceinline__ void b_g2p(Ctrl *c, const Geom &g, const PState<T> &P
     else gather(GlobSrc{gsrc});
     const T dt = T(c->dt), C = T(M.C_pf);
     T vn[D];
+    unsigned long long bm[D];
     bool bad = false;
 #pragma unroll
     for (int d = 0; d < D; d++) {
@@ -1073,10 +1095,12 @@ __device__ __forceinline__ void b_g2p(Ctrl *c, const Geom &g, const PState<T> &P
             P.x[par ^ 1][d][p] = xn;
             if (!isfinite(vn[d]) || !isfinite(xn)) bad = true;
         }
-        const unsigned long long bm =
-            block_max_bits(live ? (unsigned long long)__double_as_longlong(double(fabs(vn[d]))) : 0ull, sred);
-        if (threadIdx.x == 0) c->vmax_part[d * c->nblk + blk] = bm;
+        bm[d] = live ? (unsigned long long)__double_as_longlong(double(fabs(vn[d]))) : 0ull;
     }
+    block_max_bits_n<D>(bm, sred);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int d = 0; d < D; d++) c->vmax_part[d * c->nblk + blk] = bm[d];
     if (bad) atomicOr(&c->err, ERR_NONFINITE);
     if (M.musl) {
         const bool lds = box.fits(D);
