@@ -1616,17 +1616,23 @@ __device__ __forceinline__ T grad_v_rows(const BlockBox<3> &b, const W<3, T> &w,
 
 // ΔF = I + dt ∇v, F_new = ΔF F (deform.jl:14-15)
 template <int D, typename T>
-__device__ __forceinline__ void deform_f(const PState<T> &P, long q, T dt, const T (&gv)[D][D], T (&dF)[D][D],
-                                         T (&Fn)[D][D]) {
+__device__ __forceinline__ void delta_f(T dt, const T (&gv)[D][D], T (&dF)[D][D]) {
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+        for (int j = 0; j < D; j++)
+            // FP32 path: dF holds K = dt ∇v = ΔF - I
+            dF[i][j] = kDev<T> ? dt * gv[i][j] : (i == j ? T(1) : T(0)) + dt * gv[i][j];
+}
+
+// Fn = ΔF F (FP32: F + K F)
+template <int D, typename T>
+__device__ __forceinline__ void push_f(const PState<T> &P, long q, const T (&dF)[D][D], T (&Fn)[D][D]) {
     T F[D][D];
 #pragma unroll
     for (int i = 0; i < D; i++)
 #pragma unroll
-        for (int j = 0; j < D; j++) {
-            // FP32 path: dF holds K = dt ∇v = ΔF - I
-            dF[i][j] = kDev<T> ? dt * gv[i][j] : (i == j ? T(1) : T(0)) + dt * gv[i][j];
-            F[i][j] = P.F[i + D * j][q];
-        }
+        for (int j = 0; j < D; j++) F[i][j] = P.F[i + D * j][q];
 #pragma unroll
     for (int i = 0; i < D; i++)
 #pragma unroll
@@ -1636,6 +1642,13 @@ __device__ __forceinline__ void deform_f(const PState<T> &P, long q, T dt, const
             for (int k = 1; k < D; k++) a += dF[i][k] * F[k][j];
             Fn[i][j] = kDev<T> ? F[i][j] + a : a;
         }
+}
+
+template <int D, typename T>
+__device__ __forceinline__ void deform_f(const PState<T> &P, long q, T dt, const T (&gv)[D][D], T (&dF)[D][D],
+                                         T (&Fn)[D][D]) {
+    delta_f<D, T>(dt, gv, dF);
+    push_f<D, T>(P, q, dF, Fn);
 }
 
 // GIMP domain update (domain.jl:1-52) chosen by basis.how per framework
@@ -1816,23 +1829,15 @@ __device__ __forceinline__ void b_fbar(Ctrl *c, const Geom &g, const PState<T> &
         else grad_v<D, T>(g, w, GlobSrc{G.v}, 0, gv);
     }
     T dF[D][D], Fn[D][D];
-    deform_f<D, T>(P, p, T(c->dt), gv, dF, Fn);
+    delta_f<D, T>(T(c->dt), gv, dF);
     const T dJp = kDev<T> ? det1m<D, T>(dF) : det<D, T>(dF);
-    const T J = det<D, T>(Fn);
-#pragma unroll
-    for (int i = 0; i < D; i++)
-#pragma unroll
-        for (int j = 0; j < D; j++) {
-            P.F[i + D * j][p] = Fn[i][j];
-            if (M.tpic) P.gv[i + D * j][p] = gv[i][j];
-        }
-    P.J[p] = J;
-    if (B == 1) domain_update<D, T>(P, M, p, dF, Fn, J);
     T acc = T(0);
     // FP64: Σ N ΔJ_n/ΔJ_p as volumetric.jl:27-32 (3D: (Σ N ΔJ_n)/ΔJ_p, one
     // division instead of 64).  FP32: the nodal values are
     // δJ_n = ΔJ_n - 1 and Σ N δJ_n is accumulated, so that with Σ N = 1
     // ΔJ̄/ΔJ_p - 1 = (Σ N δJ_n - δJ_p)/(1 + δJ_p) keeps its relative precision.
+    // This is the weights' last use: done before F is read, the weights are
+    // not live beside F and ΔF F (FP64: 144 -> 130 VGPRs)
     const T rdiv = kDev<T> ? T(1) : dJp;
     if (D == 3 && staged) {
         acc = kDev<T> ? qsum : qsum / rdiv;
@@ -1846,6 +1851,17 @@ __device__ __forceinline__ void b_fbar(Ctrl *c, const Geom &g, const PState<T> &
             acc += N * (m == 0.0 ? T(0) : T(G.dJ[no] / m)) / rdiv;
         });
     }
+    push_f<D, T>(P, p, dF, Fn);
+    const T J = det<D, T>(Fn);
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+            P.F[i + D * j][p] = Fn[i][j];
+            if (M.tpic) P.gv[i + D * j][p] = gv[i][j];
+        }
+    P.J[p] = J;
+    if (B == 1) domain_update<D, T>(P, M, p, dF, Fn, J);
     if constexpr (kDev<T>) {
         const T rm1 = (acc - dJp) / (T(1) + dJp);
         const T fm1 = expm1(log1p(rm1) / T(D));   // (ΔJ̄/ΔJ_p)^(1/d) - 1
@@ -1868,10 +1884,11 @@ __device__ __forceinline__ void b_fbar(Ctrl *c, const Geom &g, const PState<T> &
 }
 
 // kernel of one context: block b of the launch is block b of the particles / nodes
-// FP32 k_fbar at 5 waves per SIMD (at most 96 VGPRs, a few spilled) runs
-// 10 % faster than at the 4 its 104 VGPRs allow; 6 waves spill too much, and
-// the FP64 form spills at 4 without gaining
-template <typename T> constexpr int kFbarWaves = kDev<T> ? 5 : 2;
+// k_fbar's register budget: FP32 at 5 waves per SIMD (at most 96 VGPRs, a
+// few spilled) runs 10 % faster than at the 4 its 104 VGPRs allow (6 waves
+// spill too much); FP64 at 4 waves (128 VGPRs, 2 spilled) 9 % faster than at
+// the 3 its 130 allow
+template <typename T> constexpr int kFbarWaves = kDev<T> ? 5 : 4;
 
 template <int D, int B, typename T, bool SURR = false>
 __global__ void __launch_bounds__(EP_PBLK_MAX) EP_WAVES_PER_EU(kFbarWaves<T>) k_fbar(Ctrl *c, Geom g, PState<T> P, GState G, Mat M) { b_fbar<D, B, T, SURR>(c, g, P, G, M, blockIdx.x); }
