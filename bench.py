@@ -58,13 +58,13 @@ KERNEL_BYTES_3D = {
     "fp64": {
         "p2g": 8 * (3 + 3 + 6 + 1 + 1 + 1),                  # x v tau J rho0 Om0
         "g2p": 8 * ((3 + 3 + 1 + 1 + 1) + (3 + 3)),          # reads x v J rho0 Om0, writes x' v
-        "deform": 8 * (3 + 9 + 1 + 1),                       # reads x F rho0 Om0 (scatters m*dJ to the grid)
+        "deform": 8 * (3 + 1 + 1 + 1),                       # reads x J rho0 Om0 (scatters m*dJ to the grid)
         "fbar": 8 * ((3 + 9 + 6 + 1 + 3) + (9 + 1 + 6 + 6 + 1 + 1)),  # reads x F b epII c0 cr phi,
     },                                                        # writes F J b tau epII dlam
     "fp32": {
         "p2g": 8 * 3 + 4 * (3 + 6 + 1 + 1 + 1),
         "g2p": 8 * (3 + 3) + 4 * ((3 + 1 + 1 + 1) + 3),
-        "deform": 8 * 3 + 4 * (9 + 1 + 1),
+        "deform": 8 * 3 + 4 * (1 + 1 + 1),
         "fbar": 8 * 3 + 4 * ((9 + 6 + 1 + 3) + (9 + 1 + 6 + 6 + 1 + 1)),
     },
 }

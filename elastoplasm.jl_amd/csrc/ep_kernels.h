@@ -1644,13 +1644,6 @@ __device__ __forceinline__ void push_f(const PState<T> &P, long q, const T (&dF)
         }
 }
 
-template <int D, typename T>
-__device__ __forceinline__ void deform_f(const PState<T> &P, long q, T dt, const T (&gv)[D][D], T (&dF)[D][D],
-                                         T (&Fn)[D][D]) {
-    delta_f<D, T>(dt, gv, dF);
-    push_f<D, T>(P, q, dF, Fn);
-}
-
 // GIMP domain update (domain.jl:1-52) chosen by basis.how per framework
 // (update.jl:7-29), after the deformation update and before F-bar
 // (update.jl:43-49).  Restated from intent: domain.jl reads mpts.Fᵢⱼ and
@@ -1735,11 +1728,17 @@ __device__ __forceinline__ void b_deform(Ctrl *c, const Geom &g, const PState<T>
         if (staged) grad_v<D, T>(g, w, LdsSrc<D, D, T>{&box}, 0, gv);
         else grad_v<D, T>(g, w, GlobSrc{G.v}, 0, gv);
     }
-    T dF[D][D], Fn[D][D];
-    deform_f<D, T>(P, q, T(c->dt), gv, dF, Fn);
-    const T J = det<D, T>(Fn);
+    T dF[D][D];
+    delta_f<D, T>(T(c->dt), gv, dF);
     if (LOCK) {
         const T dJ = kDev<T> ? det1m<D, T>(dF) : det<D, T>(dF);   // FP32 path: δJ = ΔJ - 1
+        // J of the updated F enters here only through the particle mass
+        // (ρ₀/J · JΩ₀ finite, ρ · JΩ₀ infinitesimal; deform.jl:19-21, 41-43).
+        // It is taken as J_old ΔJ, which equals det(ΔF F) up to rounding, so
+        // F is not read here (k_fbar updates it): 9 loads fewer, and no
+        // registers for F and ΔF F
+        const T Jo = P.J[q];
+        const T J = kDev<T> ? Jo + Jo * dJ : Jo * dJ;
         const bool lds = box.fits(1);
         if (lds) box.zero(1);
         double *dst[1] = {G.dJ};
@@ -1760,6 +1759,9 @@ __device__ __forceinline__ void b_deform(Ctrl *c, const Geom &g, const PState<T>
         else scatter(GlobSink{dst});
         if (lds) box.template flush<1>(g, dst);
     } else if (live) {
+        T Fn[D][D];
+        push_f<D, T>(P, q, dF, Fn);
+        const T J = det<D, T>(Fn);
 #pragma unroll
         for (int i = 0; i < D; i++)
 #pragma unroll
