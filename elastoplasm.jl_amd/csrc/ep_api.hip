@@ -109,7 +109,7 @@ struct ep_ctx {
     // periodic re-sort into brick-major cell order (k_sort_keys): slot list
     // on the device, spare field buffers of each element type, sort scratch
     // resort_every > 0: every that many steps; < 0: adaptive, the p2g box
-    // overflow count is read every -resort_every steps; 0: never
+    // counters are read every -resort_every steps; 0: never
     int resort_every = kResortAdaptive;
     long since_sort = 0;
     int spill_seen = 0;                // ctrl->spill at the last check
@@ -121,8 +121,8 @@ struct ep_ctx {
     bool in_ensemble = false;          // an ensemble holds this context's field pointers
     int *perm_d = nullptr;             // device copy of perm, permuted with the fields
     bool perm_dirty = false;
-    double *spare_d = nullptr;
-    float *spare_f = nullptr;
+    double *spare_d[kPermK] = {};
+    float *spare_f[kPermK] = {};
     int *spare_i = nullptr;
     std::vector<void *> sort_allocs;
     unsigned *sort_key[2] = {nullptr, nullptr};
@@ -157,6 +157,8 @@ struct ep_ctx {
     ep_stats stats{};
     double t_host = 0.0;
 };
+
+static int sort_buffers(ep_ctx *c);
 
 extern "C" int ep_version(void) { return EP_ABI_VERSION; }
 extern "C" const char *ep_last_error(void) { return g_err.c_str(); }
@@ -428,9 +430,11 @@ static int upload_fields(ep_ctx *c, const ep_particles_view *v, PState<T> &P) {
         c->part_allocs.push_back(a);
         return static_cast<int *>(a);
     };
-    c->spare_d = take_d();
-    c->spare_f = nullptr;
-    if constexpr (std::is_same<T, float>::value) c->spare_f = take();
+    for (int k = 0; k < kPermK; k++) {
+        c->spare_d[k] = take_d();
+        c->spare_f[k] = nullptr;
+        if constexpr (std::is_same<T, float>::value) c->spare_f[k] = take();
+    }
     c->perm_d = take_i();
     c->spare_i = take_i();
     if (oom) return fail(EP_E_HIP, "hipMalloc of particle fields failed");
@@ -537,7 +541,8 @@ static int upload_fields(ep_ctx *c, const ep_particles_view *v, PState<T> &P) {
     c->perm_dirty = false;
 #undef PUT
 #undef PUTD
-    return EP_OK;
+    // a context that re-sorts (the default) gets its re-sort buffers now
+    return c->resort_every != 0 ? sort_buffers(c) : EP_OK;
 }
 
 extern "C" int ep_upload_particles(ep_ctx *c, const ep_particles_view *v) {
@@ -865,6 +870,39 @@ static void dispatch_step_t(ep_ctx *c, const PState<T> &P) {
 // their brick-major cell key, then every particle field gathered through the
 // sorted slot list into a spare buffer that takes the field's place.  The
 // slot count stays; padding slots move to the end.
+// Keys, values, slot index and hipCUB's temporary storage of the re-sort,
+// allocated once per particle upload so that no re-sort inside a run
+// allocates.
+static int sort_buffers(ep_ctx *c) {
+    const long n = c->nslot;
+    auto alloc = [&](size_t bytes) -> void * {
+        void *a = nullptr;
+        if (hipMalloc(&a, bytes) != hipSuccess) return nullptr;
+        c->sort_allocs.push_back(a);
+        return a;
+    };
+    c->sort_key[0] = static_cast<unsigned *>(alloc(sizeof(unsigned) * n));
+    c->sort_key[1] = static_cast<unsigned *>(alloc(sizeof(unsigned) * n));
+    c->sort_val = static_cast<int *>(alloc(sizeof(int) * n));
+    c->sort_idx = static_cast<int *>(alloc(sizeof(int) * n));
+    if (!c->sort_key[0] || !c->sort_key[1] || !c->sort_val || !c->sort_idx) {
+        c->sort_idx = nullptr;
+        return fail(EP_E_HIP, "hipMalloc of the re-sort buffers failed");
+    }
+#ifndef EP_CPU_CHECK
+    size_t bytes = 0;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, c->sort_key[0], c->sort_key[1], c->sort_val,
+                                              c->sort_idx, int(n), 0, 32, c->stream));
+    c->sort_tmp = alloc(bytes);
+    c->sort_tmp_bytes = bytes;
+    if (!c->sort_tmp) {
+        c->sort_idx = nullptr;
+        return fail(EP_E_HIP, "hipMalloc of the re-sort buffers failed");
+    }
+#endif
+    return EP_OK;
+}
+
 template <int D, typename T>
 static int resort_t(ep_ctx *c, PState<T> &P) {
     const long n = c->nslot;
@@ -872,31 +910,8 @@ static int resort_t(ep_ctx *c, PState<T> &P) {
     const int TB = 256;
     const dim3 gp((n + TB - 1) / TB);
     if (!c->sort_idx) {
-        auto alloc = [&](size_t bytes) -> void * {
-            void *a = nullptr;
-            if (hipMalloc(&a, bytes) != hipSuccess) return nullptr;
-            c->sort_allocs.push_back(a);
-            return a;
-        };
-        c->sort_key[0] = static_cast<unsigned *>(alloc(sizeof(unsigned) * n));
-        c->sort_key[1] = static_cast<unsigned *>(alloc(sizeof(unsigned) * n));
-        c->sort_val = static_cast<int *>(alloc(sizeof(int) * n));
-        c->sort_idx = static_cast<int *>(alloc(sizeof(int) * n));
-        if (!c->sort_key[0] || !c->sort_key[1] || !c->sort_val || !c->sort_idx) {
-            c->sort_idx = nullptr;
-            return fail(EP_E_HIP, "hipMalloc of the re-sort buffers failed");
-        }
-#ifndef EP_CPU_CHECK
-        size_t bytes = 0;
-        HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, c->sort_key[0], c->sort_key[1], c->sort_val,
-                                                  c->sort_idx, int(n), 0, 32, s));
-        c->sort_tmp = alloc(bytes);
-        c->sort_tmp_bytes = bytes;
-        if (!c->sort_tmp) {
-            c->sort_idx = nullptr;
-            return fail(EP_E_HIP, "hipMalloc of the re-sort buffers failed");
-        }
-#endif
+        const int r = sort_buffers(c);
+        if (r != EP_OK) return r;
     }
     EP_LAUNCH((k_sort_keys<D, T>), gp, TB, s, c->ctrl, c->geom, P, n, int(c->brick[0]), int(c->brick[1]),
               int(c->brick[2]), c->sort_key[0], c->sort_val);
@@ -913,35 +928,57 @@ static int resort_t(ep_ctx *c, PState<T> &P) {
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(c->sort_tmp, c->sort_tmp_bytes, c->sort_key[0], c->sort_key[1],
                                               c->sort_val, c->sort_idx, int(n), 0, 32, s));
 #endif
-    T *&spare = [&]() -> T *& {
-        if constexpr (std::is_same<T, double>::value) return c->spare_d;
-        else return c->spare_f;
-    }();
-    auto gather = [&](auto *&f, auto *&sp) {
+    // every field into sorted order, kPermK of one element type per launch
+    // (the slot index read once per group); each spare then takes its
+    // field's place
+    std::vector<double **> fd;
+    std::vector<float **> ff;
+    std::vector<int **> fi;
+    auto add = [&](auto *&f) {
         if (!f) return;
         using E = std::remove_pointer_t<std::remove_reference_t<decltype(f)>>;
-        EP_LAUNCH((k_permute<E>), gp, TB, s, f, sp, c->sort_idx, n);
-        std::swap(f, sp);
+        if constexpr (std::is_same<E, double>::value) fd.push_back(&f);
+        else if constexpr (std::is_same<E, float>::value) ff.push_back(&f);
+        else fi.push_back(&f);
     };
     constexpr int DD = D * D, NB = D == 2 ? 3 : 6;
     for (int k = 0; k < 2; k++)
-        for (int i = 0; i < D; i++) gather(P.x[k][i], c->spare_d);
-    for (int i = 0; i < D; i++) gather(P.v[i], spare);
-    for (int i = 0; i < DD; i++) gather(P.F[i], spare);
-    for (int i = 0; i < NB; i++) gather(P.b[i], spare);
-    for (int i = 0; i < NB; i++) gather(P.s[i], spare);
-    gather(P.epII, spare);
-    gather(P.dlam, spare);
-    gather(P.J, spare);
-    gather(P.rho, spare);
-    gather(P.Om0, spare);
-    gather(P.c0, spare);
-    gather(P.cr, spare);
-    gather(P.phi, spare);
-    for (int i = 0; i < D; i++) { gather(P.l[i], spare); gather(P.l0[i], spare); }
-    for (int i = 0; i < DD; i++) { gather(P.gv[i], spare); gather(P.Bm[i], spare); }
-    gather(P.epII2, spare);
-    gather(c->perm_d, c->spare_i);
+        for (int i = 0; i < D; i++) add(P.x[k][i]);
+    for (int i = 0; i < D; i++) add(P.v[i]);
+    for (int i = 0; i < DD; i++) add(P.F[i]);
+    for (int i = 0; i < NB; i++) add(P.b[i]);
+    for (int i = 0; i < NB; i++) add(P.s[i]);
+    add(P.epII);
+    add(P.dlam);
+    add(P.J);
+    add(P.rho);
+    add(P.Om0);
+    add(P.c0);
+    add(P.cr);
+    add(P.phi);
+    for (int i = 0; i < D; i++) { add(P.l[i]); add(P.l0[i]); }
+    for (int i = 0; i < DD; i++) { add(P.gv[i]); add(P.Bm[i]); }
+    add(P.epII2);
+    add(c->perm_d);
+    auto permute = [&](auto &fields, auto *const *spares) {
+        using E = std::remove_pointer_t<std::remove_pointer_t<typename std::remove_reference_t<decltype(fields)>::value_type>>;
+        for (size_t g0 = 0; g0 < fields.size(); g0 += kPermK) {
+            PermArgs<E> a{};
+            a.cnt = int(std::min<size_t>(kPermK, fields.size() - g0));
+            for (int k = 0; k < a.cnt; k++) {
+                a.src[k] = *fields[g0 + k];
+                a.dst[k] = spares[k];
+            }
+            EP_LAUNCH((k_permute<E>), gp, TB, s, a, c->sort_idx, n);
+            for (int k = 0; k < a.cnt; k++) std::swap(*fields[g0 + k], const_cast<E *&>(spares[k]));
+        }
+    };
+    permute(fd, c->spare_d);
+    permute(ff, c->spare_f);
+    permute(fi, &c->spare_i);
+    // the non-local pass reads each particle's original index through
+    // nl.orig, which must follow perm_d into its new buffer
+    c->mat.nl.orig = c->perm_d;
     c->perm_dirty = true;
     return EP_OK;
 }

@@ -1236,10 +1236,24 @@ __global__ void __launch_bounds__(256) k_sort_keys(const Ctrl *c, Geom g, PState
     val[p] = int(p);
 }
 
+// up to kPermK particle fields of one element type gathered into sorted
+// order by one launch: dst[k][i] = src[k][idx[i]]
+constexpr int kPermK = 6;
 template <typename E>
-__global__ void __launch_bounds__(256) k_permute(const E *src, E *dst, const int *idx, long n) {
+struct PermArgs {
+    const E *src[kPermK];
+    E *dst[kPermK];
+    int cnt;
+};
+
+template <typename E>
+__global__ void __launch_bounds__(256) k_permute(PermArgs<E> a, const int *idx, long n) {
     const long i = blockIdx.x * long(blockDim.x) + threadIdx.x;
-    if (i < n) dst[i] = src[idx[i]];
+    if (i >= n) return;
+    const long j = idx[i];
+#pragma unroll
+    for (int k = 0; k < kPermK; k++)
+        if (k < a.cnt) a.dst[k][i] = a.src[k][j];
 }
 
 // augm_velocity (augm.jl:37-53) after mesh.v .= 0 (n2p.jl:67)

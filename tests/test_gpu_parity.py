@@ -26,9 +26,11 @@ C1 = dict(rho0=2650.0, E=1.0e6, nu=0.3, c0=20.0e3, cr=20.0e3, phi0=math.radians(
           phir=math.radians(30.0), Hp=0.0)
 
 
-def _device_run(ic, cfg, nsteps, T, *, t0=0.0, plast=True, ramp=False, tg=0.0, record=False):
+def _device_run(ic, cfg, nsteps, T, *, t0=0.0, plast=True, ramp=False, tg=0.0, record=False, resort=None):
     dev = hip.Device(ic.mesh, ic.mpts, ic.cmpr, cfg.instr)
     try:
+        if resort is not None:
+            dev.set_resort(resort)
         dev.set_time(t0)
         r = dev.step(nsteps, T, plast=plast, ramp=ramp, tg=tg, record_dt=record)
         dts = dev.dts(nsteps) if record else None
@@ -262,18 +264,21 @@ def test_resume_through_host_state(dim):
     assert max(compare(two.mpts, ref.mpts, fields).values()) <= RUN_TOL
 
 
-@pytest.mark.parametrize("dim,nsteps,ls", [(2, 80, 2.0), (2, 40, 5.0), (3, 25, 2.0)])
-def test_nonlocal_regularisation(dim, nsteps, ls):
+@pytest.mark.parametrize("dim,nsteps,ls,resort", [(2, 80, 2.0, None), (2, 40, 5.0, None), (3, 25, 2.0, None),
+                                                    (2, 40, 2.0, 3)])
+def test_nonlocal_regularisation(dim, nsteps, ls, resort):
     """Non-local plastic strain (nonlocal.jl:18-47, plast.jl:49-60) with the
     reference's softening material (c0 20 kPa -> cr 4 kPa, Hp -30 kPa, weak
-    layer φr 7.5°), so that ϵpII[2] enters the cohesion of the return."""
+    layer φr 7.5°), so that ϵpII[2] enters the cohesion of the return.  The
+    last case re-sorts every 3 steps: the pass excludes pairs by the
+    particles' original order, which must follow the particles."""
     ic, cfg = slump(dim, C_pf=1.0)
     cfg = clone(cfg)
     cfg.instr["nonloc"] = {"status": True, "ls": ls}
     x0 = ic.mpts.x.copy(order="F")
     ref, dev = clone(ic), clone(ic)
     Oracle(ref.mesh, ref.mpts, ref.cmpr, cfg.instr).run(nsteps, 100.0)
-    r, _ = _device_run(dev, cfg, nsteps, 100.0)
+    r, _ = _device_run(dev, cfg, nsteps, 100.0, resort=resort)
     assert r.steps == nsteps
     a, b = dev.mpts, ref.mpts
     npl = int((b.s.dlam != 0).sum())
