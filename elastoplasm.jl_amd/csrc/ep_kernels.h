@@ -1478,15 +1478,21 @@ __device__ __forceinline__ void constitutive(Ctrl *c, const PState<T> &P, const 
         }
         if (plast && M.plast == 1) {
             T e1 = P.epII[p], e2 = M.nonlocal ? P.epII2[p] : e1, dl;
-            dp_return<NS, T>(sp, P.c0[p], P.cr[p], P.phi[p], T(M.Hp), T(M.Kc), T(M.Gc), e2, e1, dl);
+            const int br = dp_return<NS, T>(sp, P.c0[p], P.cr[p], P.phi[p], T(M.Hp), T(M.Kc), T(M.Gc), e2, e1, dl);
             P.dlam[p] = dl;
             P.epII[p] = e1;
-            T y[NS];
-            lu_solve<NS, T>(M.lu, sp, y);
-            T fe[D];
+            // b from the returned stress, exp(2 Del⁻¹τ) (DP.jl:63-66).  An
+            // elastic trial keeps the trial b, which that formula gives back
+            // up to rounding (Del⁻¹ Del ε = ε, exp 2ε = the eigenvalues of b)
+            // at the cost of a 6x6 solve and d exponentials per particle
+            if (br != 0) {
+                T y[NS];
+                lu_solve<NS, T>(M.lu, sp, y);
+                T fe[D];
 #pragma unroll
-            for (int k = 0; k < D; k++) fe[k] = kDev<T> ? expm1(T(2) * y[k]) : exp(T(2) * y[k]);
-            vdvt<D, T>(V, fe, bn);
+                for (int k = 0; k < D; k++) fe[k] = kDev<T> ? expm1(T(2) * y[k]) : exp(T(2) * y[k]);
+                vdvt<D, T>(V, fe, bn);
+            }
         }
         T tp[D];
 #pragma unroll
