@@ -418,6 +418,7 @@ static int upload_fields(ep_ctx *c, const ep_particles_view *v, PState<T> &P) {
     for (int i = 0; i < NS; i++) P.s[i] = take();
     P.epII = take(); P.dlam = take(); P.J = take(); P.rho = take();
     P.Om0 = take(); P.c0 = take(); P.cr = take(); P.phi = take();
+    P.sphi = take(); P.cphi = take();
     if (c->cfg.basis == EP_GIMPM) for (int i = 0; i < D; i++) P.l[i] = take();
     if (M.domain != EP_UNDEFORMED) for (int i = 0; i < D; i++) P.l0[i] = take();
     if (M.tpic) for (int i = 0; i < DD; i++) P.gv[i] = take();
@@ -529,6 +530,12 @@ static int upload_fields(ep_ctx *c, const ep_particles_view *v, PState<T> &P) {
         for (long q = 0; q < n; q++) pi[q] = int(c->perm[q]);
         if (hipMemcpy(c->perm_d, pi.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess)
             return fail(EP_E_HIP, "hipMemcpy particle upload failed");
+    }
+    {
+        const dim3 gf((n + 255) / 256);
+        if (D == 2) EP_LAUNCH((k_friction<3, T>), gf, 256, c->stream, P.phi, P.sphi, P.cphi, n);
+        else EP_LAUNCH((k_friction<6, T>), gf, 256, c->stream, P.phi, P.sphi, P.cphi, n);
+        HIPCHK(hipGetLastError());
     }
     for (void *a : c->sort_allocs) hipFree(a);
     c->sort_allocs.clear();
@@ -956,6 +963,8 @@ static int resort_t(ep_ctx *c, PState<T> &P) {
     add(P.c0);
     add(P.cr);
     add(P.phi);
+    add(P.sphi);
+    add(P.cphi);
     for (int i = 0; i < D; i++) { add(P.l[i]); add(P.l0[i]); }
     for (int i = 0; i < DD; i++) { add(P.gv[i]); add(P.Bm[i]); }
     add(P.epII2);

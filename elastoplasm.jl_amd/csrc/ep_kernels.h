@@ -42,6 +42,7 @@ struct PState {
     T *epII, *dlam, *J;
     T *rho;                    // ρ₀ (finite: ρ = ρ₀/J) or the constant ρ (infinitesimal)
     T *Om0, *c0, *cr, *phi;
+    T *sphi, *cphi;            // tan φ (2D) or sin φ, cos φ (3D) of the constant φ, set at upload
     T *l[3];                   // GIMP half lengths ℓ the basis uses
     T *l0[3];                  // ℓ₀, kept for domain updates other than `undeformed`
     T *gv[9];                  // ∇v (TPIC state)
@@ -1236,6 +1237,18 @@ __global__ void __launch_bounds__(256) k_sort_keys(const Ctrl *c, Geom g, PState
     val[p] = int(p);
 }
 
+// the friction angle's trigonometric values the Drucker-Prager return reads,
+// once per upload (φ does not change during a run)
+template <int NSTR, typename T>
+__global__ void __launch_bounds__(256) k_friction(const T *phi, T *sphi, T *cphi, long n) {
+    const long i = blockIdx.x * long(blockDim.x) + threadIdx.x;
+    if (i >= n) return;
+    T sf, cf;
+    friction_trig<NSTR, T>(phi[i], sf, cf);
+    sphi[i] = sf;
+    cphi[i] = cf;
+}
+
 // up to kPermK particle fields of one element type gathered into sorted
 // order by one launch: dst[k][i] = src[k][idx[i]]
 constexpr int kPermK = 6;
@@ -1446,7 +1459,7 @@ __device__ __forceinline__ void constitutive(Ctrl *c, const PState<T> &P, const 
             // writes τ, b, ϵpII and Δλ; the queue slots are reserved with one
             // atomic per wave.
             const bool pl = plast && M.plast == 1 &&
-                            dp_trial<NS, T>(sp, P.c0[p], P.cr[p], P.phi[p], T(M.Hp), P.epII[p]) != 0;
+                            dp_trial_trig<NS, T>(sp, P.c0[p], P.cr[p], P.sphi[p], P.cphi[p], T(M.Hp), P.epII[p]) != 0;
             const unsigned long long mask = __ballot(pl);
             if (mask) {
                 const int lane = int(__lane_id());
@@ -1478,7 +1491,8 @@ __device__ __forceinline__ void constitutive(Ctrl *c, const PState<T> &P, const 
         }
         if (plast && M.plast == 1) {
             T e1 = P.epII[p], e2 = M.nonlocal ? P.epII2[p] : e1, dl;
-            const int br = dp_return<NS, T>(sp, P.c0[p], P.cr[p], P.phi[p], T(M.Hp), T(M.Kc), T(M.Gc), e2, e1, dl);
+            const int br = dp_return_trig<NS, T>(sp, P.c0[p], P.cr[p], P.sphi[p], P.cphi[p], T(M.Hp), T(M.Kc),
+                                                 T(M.Gc), e2, e1, dl);
             P.dlam[p] = dl;
             P.epII[p] = e1;
             // b from the returned stress, exp(2 Del⁻¹τ) (DP.jl:63-66).  An
@@ -1546,7 +1560,7 @@ __device__ __forceinline__ void constitutive(Ctrl *c, const PState<T> &P, const 
         }
         if (c->with_plast && M.plast == 1) {
             T e1 = P.epII[p], e2 = M.nonlocal ? P.epII2[p] : e1, dl;
-            dp_return<NS, T>(s, P.c0[p], P.cr[p], P.phi[p], T(M.Hp), T(M.Kc), T(M.Gc), e2, e1, dl);
+            dp_return_trig<NS, T>(s, P.c0[p], P.cr[p], P.sphi[p], P.cphi[p], T(M.Hp), T(M.Kc), T(M.Gc), e2, e1, dl);
             P.dlam[p] = dl;
             P.epII[p] = e1;
         } else if (c->with_plast && M.plast == 2) {

@@ -297,9 +297,18 @@ __device__ __forceinline__ void lu_solve(const DelLU &f, const T *b, T *x) {
 // The step passes ψ = 0 as finite_DP / infinitesimal_DP hard-code it
 // (DP.jl:37, 73); the surrogate's training samples use materialParam's ψ
 // argument (DP.jl:13-24) for ψ in [0, φ].
+// The friction angle enters through sf = tan φ (2D) or sin φ (3D) and
+// cf = cos φ (3D); φ is constant per particle, so the step computes them once
+// per particle (friction_trig) and the return below reads them.
 template <int NSTR, typename T>
-__device__ __forceinline__ int dp_return(T *s, T c0, T cr, T phi, T Hp, T Kc, T Gc, T epII2, T &epII1, T &dlam,
-                                         T psi = T(0)) {
+__device__ __forceinline__ void friction_trig(T phi, T &sf, T &cf) {
+    if (NSTR == 3) { sf = tan(phi); cf = T(0); }
+    else { sf = sin(phi); cf = cos(phi); }
+}
+
+template <int NSTR, typename T>
+__device__ __forceinline__ int dp_return_trig(T *s, T c0, T cr, T sf, T cf, T Hp, T Kc, T Gc, T epII2, T &epII1,
+                                              T &dlam, T psi = T(0)) {
     constexpr int ND = NSTR == 3 ? 2 : 3;
     dlam = T(0);
     T c = c0 + Hp * epII2;
@@ -309,7 +318,7 @@ __device__ __forceinline__ int dp_return(T *s, T c0, T cr, T phi, T Hp, T Kc, T 
         P = (s[0] + s[1]) / T(2);
         t0[0] = s[0] - P; t0[1] = s[1] - P; t0[2] = s[2] - T(0);
         tII = sqrt(T(0.5) * (t0[0] * t0[0] + t0[1] * t0[1]) + t0[2] * t0[2]);
-        const T tp = tan(phi), tq = tan(psi);
+        const T tp = sf, tq = tan(psi);
         eta = T(3) * tp / (sqrt(T(9) + T(12) * tp * tp));
         etaB = T(3) * tq / (sqrt(T(9) + T(12) * tq * tq));
         xi = T(3) * c / (sqrt(T(9) + T(12) * tp * tp));
@@ -320,10 +329,10 @@ __device__ __forceinline__ int dp_return(T *s, T c0, T cr, T phi, T Hp, T Kc, T 
 #pragma unroll
         for (int k = 3; k < NSTR; k++) t0[k] = s[k] - T(0);
         tII = sqrt(T(0.5) * (t0[0] * t0[0] + t0[1] * t0[1] + t0[2] * t0[2]) + t0[3] * t0[3] + t0[4] * t0[4] + t0[5] * t0[5]);
-        const T sp = sin(phi), sq = sin(psi);
+        const T sp = sf, sq = sin(psi);
         eta = T(6) * sp / (sqrt(T(3)) * (T(3) + sp));
         etaB = T(6) * sq / (sqrt(T(3)) * (T(3) + sq));
-        xi = T(6) * c * cos(phi) / (sqrt(T(3)) * (T(3) + sp));
+        xi = T(6) * c * cf / (sqrt(T(3)) * (T(3) + sp));
     }
     const T sm = xi / eta, tP = xi - eta * (xi / eta);
     const T fs = tII + eta * P - xi;
@@ -352,10 +361,18 @@ __device__ __forceinline__ int dp_return(T *s, T c0, T cr, T phi, T Hp, T Kc, T 
     return branch;
 }
 
+template <int NSTR, typename T>
+__device__ __forceinline__ int dp_return(T *s, T c0, T cr, T phi, T Hp, T Kc, T Gc, T epII2, T &epII1, T &dlam,
+                                         T psi = T(0)) {
+    T sf, cf;
+    friction_trig<NSTR, T>(phi, sf, cf);
+    return dp_return_trig<NSTR, T>(s, c0, cr, sf, cf, Hp, Kc, Gc, epII2, epII1, dlam, psi);
+}
+
 // The branch dp_return would take on s (0 elastic, 1 shear, 2 apex), with the
 // same arithmetic and ψ = 0, without changing anything.
 template <int NSTR, typename T>
-__device__ __forceinline__ int dp_trial(const T *s, T c0, T cr, T phi, T Hp, T epII2) {
+__device__ __forceinline__ int dp_trial_trig(const T *s, T c0, T cr, T sf, T cf, T Hp, T epII2) {
     T c = c0 + Hp * epII2;
     if (c < cr) c = cr;
     T P, t0[NSTR], tII, eta, xi;
@@ -363,7 +380,7 @@ __device__ __forceinline__ int dp_trial(const T *s, T c0, T cr, T phi, T Hp, T e
         P = (s[0] + s[1]) / T(2);
         t0[0] = s[0] - P; t0[1] = s[1] - P; t0[2] = s[2] - T(0);
         tII = sqrt(T(0.5) * (t0[0] * t0[0] + t0[1] * t0[1]) + t0[2] * t0[2]);
-        const T tp = tan(phi);
+        const T tp = sf;
         eta = T(3) * tp / (sqrt(T(9) + T(12) * tp * tp));
         xi = T(3) * c / (sqrt(T(9) + T(12) * tp * tp));
     } else {
@@ -373,9 +390,9 @@ __device__ __forceinline__ int dp_trial(const T *s, T c0, T cr, T phi, T Hp, T e
 #pragma unroll
         for (int k = 3; k < NSTR; k++) t0[k] = s[k] - T(0);
         tII = sqrt(T(0.5) * (t0[0] * t0[0] + t0[1] * t0[1] + t0[2] * t0[2]) + t0[3] * t0[3] + t0[4] * t0[4] + t0[5] * t0[5]);
-        const T sp = sin(phi);
+        const T sp = sf;
         eta = T(6) * sp / (sqrt(T(3)) * (T(3) + sp));
-        xi = T(6) * c * cos(phi) / (sqrt(T(3)) * (T(3) + sp));
+        xi = T(6) * c * cf / (sqrt(T(3)) * (T(3) + sp));
     }
     const T sm = xi / eta, tP = xi - eta * (xi / eta);
     const T fs = tII + eta * P - xi;
@@ -383,6 +400,13 @@ __device__ __forceinline__ int dp_trial(const T *s, T c0, T cr, T phi, T Hp, T e
     if ((fs > T(0) && P < sm) || (h > T(0) && P >= sm)) return 1;
     if (h <= T(0) && P >= sm) return 2;
     return 0;
+}
+
+template <int NSTR, typename T>
+__device__ __forceinline__ int dp_trial(const T *s, T c0, T cr, T phi, T Hp, T epII2) {
+    T sf, cf;
+    friction_trig<NSTR, T>(phi, sf, cf);
+    return dp_trial_trig<NSTR, T>(s, c0, cr, sf, cf, Hp, epII2);
 }
 
 // get_J2 / yield_J2 (J2.jl:1-20)
