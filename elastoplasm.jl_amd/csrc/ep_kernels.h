@@ -1765,7 +1765,7 @@ __device__ __forceinline__ void b_deform(Ctrl *c, const Geom &g, const PState<T>
     T dF[D][D];
     delta_f<D, T>(T(c->dt), gv, dF);
     if (LOCK) {
-        const T dJ = kDev<T> ? det1m<D, T>(dF) : det<D, T>(dF);   // FP32 path: δJ = ΔJ - 1
+        const T dJ = kDev<T> ? det1m<D, T>(dF) : det_cof<D, T>(dF);   // FP32 path: δJ = ΔJ - 1
         // J of the updated F enters here only through the particle mass
         // (ρ₀/J · JΩ₀ finite, ρ · JΩ₀ infinitesimal; deform.jl:19-21, 41-43).
         // It is taken as J_old ΔJ, which equals det(ΔF F) up to rounding, so
@@ -1795,7 +1795,7 @@ __device__ __forceinline__ void b_deform(Ctrl *c, const Geom &g, const PState<T>
     } else if (live) {
         T Fn[D][D];
         push_f<D, T>(P, q, dF, Fn);
-        const T J = det<D, T>(Fn);
+        const T J = det_cof<D, T>(Fn);
 #pragma unroll
         for (int i = 0; i < D; i++)
 #pragma unroll
@@ -1866,7 +1866,7 @@ __device__ __forceinline__ void b_fbar(Ctrl *c, const Geom &g, const PState<T> &
     }
     T dF[D][D], Fn[D][D];
     delta_f<D, T>(T(c->dt), gv, dF);
-    const T dJp = kDev<T> ? det1m<D, T>(dF) : det<D, T>(dF);
+    const T dJp = kDev<T> ? det1m<D, T>(dF) : det_cof<D, T>(dF);
     T acc = T(0);
     // FP64: Σ N ΔJ_n/ΔJ_p as volumetric.jl:27-32 (3D: (Σ N ΔJ_n)/ΔJ_p, one
     // division instead of 64).  FP32: the nodal values are
@@ -1888,7 +1888,7 @@ __device__ __forceinline__ void b_fbar(Ctrl *c, const Geom &g, const PState<T> &
         });
     }
     push_f<D, T>(P, p, dF, Fn);
-    const T J = det<D, T>(Fn);
+    const T J = det_cof<D, T>(Fn);
 #pragma unroll
     for (int i = 0; i < D; i++)
 #pragma unroll

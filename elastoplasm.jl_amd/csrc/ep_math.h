@@ -195,6 +195,17 @@ __device__ __forceinline__ T det(const T (&A)[D][D]) {
     return (swaps & 1) ? -P : P;
 }
 
+// det by cofactors, for the deformation gradients (ΔF ≈ I, F with J > 0),
+// where it equals the LU determinant above up to rounding at a fraction of
+// its instructions (no pivot search, no division)
+template <int D, typename T>
+__device__ __forceinline__ T det_cof(const T (&A)[D][D]) {
+    if (D == 2) return A[0][0] * A[1][1] - A[0][1] * A[1][0];
+    return A[0][0] * (A[1][1] * A[D - 1][D - 1] - A[1][D - 1] * A[D - 1][1]) -
+           A[0][1] * (A[1][0] * A[D - 1][D - 1] - A[1][D - 1] * A[D - 1][0]) +
+           A[0][D - 1] * (A[1][0] * A[D - 1][1] - A[1][1] * A[D - 1][0]);
+}
+
 // inv (p2n.jl:243-249): LU with partial pivoting as det above, then the
 // columns of the identity by forward and back substitution.
 template <int D>
