@@ -1418,7 +1418,7 @@ __device__ __forceinline__ void constitutive(Ctrl *c, const PState<T> &P, const 
         for (int i = 0; i < D; i++)
 #pragma unroll
             for (int j = 0; j < D; j++) a[i][j] = bn[i][j];
-        sym_eig<D, T>(a, V);
+        sym_eig<D, T, !kDev<T>>(a, V);
         if constexpr (std::is_same<T, float>::value) {
             // FP64 island of the FP32 path: near-repeated eigenvalues of
             // B = b - I are decomposed again in FP64.

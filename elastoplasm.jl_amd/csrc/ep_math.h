@@ -84,13 +84,19 @@ __device__ __forceinline__ void linear(T dx, T h, T &N, T &dN) {
 // Cyclic Jacobi eigen-decomposition of a symmetric matrix; a is overwritten
 // by the rotations, v receives eigenvectors (columns).  Stands in for the
 // symmetric `eigen` of elast.jl:51 / DP.jl:65.
-template <int D, typename T>
+// FAST (the FP64 step's decomposition of the trial b, whose entries are
+// O(1)): the tangent from d = a_qq - a_pp with one division,
+// t = sgn(d) 2 a_pq / (|d| + sqrt(d² + 4 a_pq²)), and no confirming sweep after
+// a sweep whose rotations all had |t| < 1e-9 (the off-diagonal left is then
+// of order t² of the diagonal, below `tiny`).
+template <int D, typename T, bool FAST = false>
 __device__ __forceinline__ void sym_eig(T (&a)[D][D], T (&v)[D][D]) {
     for (int i = 0; i < D; i++)
         for (int j = 0; j < D; j++) v[i][j] = (i == j) ? T(1) : T(0);
     const T tiny = sizeof(T) == 8 ? T(1e-18) : T(1e-9);
     for (int sweep = 0; sweep < 32; sweep++) {
         bool rotated = false;
+        T tmax = T(0);
 #pragma unroll
         for (int p = 0; p < D - 1; p++) {
 #pragma unroll
@@ -106,10 +112,17 @@ __device__ __forceinline__ void sym_eig(T (&a)[D][D], T (&v)[D][D]) {
                 // t² + 2θt - 1 = 0 with θ = (a_qq - a_pp)/(2 a_pq).  θ is a
                 // ratio, so tiny entries (the FP32 path decomposes b - I) do
                 // not underflow as β² + a_pq² would
-                const T theta = (a[q][q] - a[p][p]) / (T(2) * apq);
                 T t;
-                if (fabs(theta) > T(sizeof(T) == 8 ? 1e150 : 1e18)) t = T(0.5) / theta;
-                else t = (theta >= T(0) ? T(1) : T(-1)) / (fabs(theta) + sqrt(theta * theta + T(1)));
+                if constexpr (FAST) {
+                    const T d = a[q][q] - a[p][p];
+                    const T r = sqrt(d * d + T(4) * apq * apq);
+                    t = (T(2) * apq) / (d >= T(0) ? d + r : d - r);
+                    tmax = fmax(tmax, fabs(t));
+                } else {
+                    const T theta = (a[q][q] - a[p][p]) / (T(2) * apq);
+                    if (fabs(theta) > T(sizeof(T) == 8 ? 1e150 : 1e18)) t = T(0.5) / theta;
+                    else t = (theta >= T(0) ? T(1) : T(-1)) / (fabs(theta) + sqrt(theta * theta + T(1)));
+                }
                 const T c = rsqrt(t * t + T(1)), s = t * c;
                 // symmetric two-sided update: the pivot pair in closed form,
                 // the other entries of rows/columns p and q once
@@ -132,7 +145,7 @@ __device__ __forceinline__ void sym_eig(T (&a)[D][D], T (&v)[D][D]) {
                 }
             }
         }
-        if (!rotated) break;
+        if (!rotated || (FAST && tmax < T(1e-9))) break;
     }
 }
 

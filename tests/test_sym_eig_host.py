@@ -3,7 +3,10 @@ for the host with g++: eigenvalues and eigenvectors against numpy for
 random symmetric matrices in FP64 and FP32, and finite results for matrices
 with tiny entries and equal diagonals, where a rotation formula built on
 β² + a_pq² underflows (the FP32 path decomposes b - I, whose entries can be
-that small)."""
+that small).  The FP64 step's fast form (one division per rotation, no
+confirming sweep) is held to the same bars on random matrices and on
+left Cauchy-Green tensors b = F Fᵀ near the identity, including repeated
+eigenvalues."""
 from __future__ import annotations
 
 import shutil
@@ -21,14 +24,14 @@ PROG = r"""
 #include "ep_cpu_check.h"
 #include "ep_math.h"
 #include <cstdio>
-template <int D, typename T> void run() {
+template <int D, typename T, bool FAST = false> void run() {
     int n;
     if (scanf("%d", &n) != 1) return;
     for (int c = 0; c < n; c++) {
         T a[D][D], v[D][D];
         for (int i = 0; i < D; i++)
             for (int j = 0; j < D; j++) { double x; if (scanf("%lf", &x) != 1) return; a[i][j] = T(x); }
-        ep::sym_eig<D, T>(a, v);
+        ep::sym_eig<D, T, FAST>(a, v);
         for (int i = 0; i < D; i++) printf("%.17g ", double(a[i][i]));
         for (int i = 0; i < D; i++)
             for (int j = 0; j < D; j++) printf("%.17g ", double(v[i][j]));
@@ -37,9 +40,9 @@ template <int D, typename T> void run() {
 }
 int main(int argc, char **argv) {
     const int d = argv[1][0] - '0';
-    const bool f = argv[2][0] == 'f';
-    if (d == 2) { if (f) run<2, float>(); else run<2, double>(); }
-    else { if (f) run<3, float>(); else run<3, double>(); }
+    const char p = argv[2][0];   // f: FP32, d: FP64, x: FP64 fast form
+    if (d == 2) { if (p == 'f') run<2, float>(); else if (p == 'x') run<2, double, true>(); else run<2, double>(); }
+    else { if (p == 'f') run<3, float>(); else if (p == 'x') run<3, double, true>(); else run<3, double>(); }
 }
 """
 
@@ -65,14 +68,14 @@ def _eig(exe, mats, dim, prec):
 
 
 @pytest.mark.parametrize("dim", [2, 3])
-@pytest.mark.parametrize("prec", ["d", "f"])
+@pytest.mark.parametrize("prec", ["d", "f", "x"])
 def test_random_symmetric(exe, dim, prec):
     rng = np.random.default_rng(31 + dim)
     mats = []
     for _ in range(40):
         A = rng.normal(size=(dim, dim))
         mats.append(0.5 * (A + A.T) * 10.0 ** rng.uniform(-6, 2))
-    tol = 1e-13 if prec == "d" else 2e-6
+    tol = 2e-6 if prec == "f" else 1e-13
     for A, (lam, V) in zip(mats, _eig(exe, mats, dim, prec)):
         scale = np.abs(A).max()
         assert np.allclose(np.sort(lam), np.linalg.eigvalsh(A), atol=tol * scale)
@@ -93,3 +96,24 @@ def test_tiny_entries_stay_finite(exe, dim, prec):
     for A, (lam, V) in zip(mats, _eig(exe, mats, dim, prec)):
         assert np.isfinite(lam).all() and np.isfinite(V).all(), A
         assert np.abs(V.T @ V - np.eye(dim)).max() <= 1e-5
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_fast_form_on_trial_b(exe, dim):
+    """b = F Fᵀ with F = I + small (the step's trial b): the fast form's
+    log(b) = V diag(log λ) Vᵀ matches scipy's matrix logarithm, also for
+    repeated eigenvalues (pure stretches along the axes, F = diag)."""
+    from scipy.linalg import logm
+    rng = np.random.default_rng(7 + dim)
+    mats = []
+    for k in range(60):
+        if k % 3 == 0:
+            F = np.diag(1.0 + rng.normal(scale=1e-3, size=dim).round(4))
+            F[0, 0] = F[-1, -1]   # a repeated eigenvalue
+        else:
+            F = np.eye(dim) + rng.normal(scale=10.0 ** rng.uniform(-6, -1), size=(dim, dim))
+        mats.append(F @ F.T)
+    for b, (lam, V) in zip(mats, _eig(exe, mats, dim, "x")):
+        L = V @ np.diag(np.log(lam)) @ V.T
+        assert np.abs(L - logm(b).real).max() <= 1e-13, b
+        assert np.abs(V.T @ V - np.eye(dim)).max() <= 1e-13
