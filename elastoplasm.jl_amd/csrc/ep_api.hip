@@ -190,6 +190,26 @@ extern "C" int ep_create(const ep_config *cfg, ep_ctx **out) {
     if (cfg->precision == EP_FP32_SURROGATE &&
         (cfg->dim != 3 || cfg->deform != EP_FINITE || cfg->plast != EP_DP))
         return fail(EP_E_CONFIG, "the learned return needs 3D finite deformation with Drucker-Prager");
+    if (cfg->deform == EP_FINITE) {
+        // the finite-strain update works in the principal frame of the trial
+        // b, which needs an isotropic Del (the reference's, from E and ν):
+        // equal normal diagonal a, equal normal off-diagonal b, shear (a-b)/2
+        // on the diagonal, zeros elsewhere
+        const int ns = cfg->dim == 2 ? 3 : 6, nd = cfg->dim;
+        const double *Dl = cfg->Del;
+        const double a = Dl[0], b = Dl[1], tol = 1e-12 * std::fabs(a);
+        bool iso = a > 0.0;
+        for (int i = 0; i < ns && iso; i++)
+            for (int j = 0; j < ns; j++) {
+                const double v = Dl[i + ns * j];
+                double want = 0.0;
+                if (i < nd && j < nd) want = i == j ? a : b;
+                else if (i == j) want = 0.5 * (a - b);
+                const bool zero = !(i < nd && j < nd) && i != j;
+                if (zero ? v != 0.0 : !(std::fabs(v - want) <= tol)) { iso = false; break; }
+            }
+        if (!iso) return fail(EP_E_CONFIG, "finite deformation needs an isotropic Del (the principal-frame update)");
+    }
     if (cfg->nmp <= 0) return fail(EP_E_ARG, "nmp must be positive");
     if (cfg->nmp > 0x7fffffffL) return fail(EP_E_ARG, "nmp exceeds 2^31 - 1");
     for (int i = 0; i < cfg->dim; i++) {

@@ -1445,12 +1445,15 @@ __device__ __forceinline__ void constitutive(Ctrl *c, const PState<T> &P, const 
         T ep[D], sp[NS];
 #pragma unroll
         for (int k = 0; k < D; k++) ep[k] = kDev<T> ? T(0.5) * log1p(a[k][k]) : T(0.5) * log(a[k][k]);
-        // principal Kirchhoff stress: Del on [ε1, ε2(, ε3), 0...]
+        // principal Kirchhoff stress: Del on [ε1, ε2(, ε3), 0...]; the shear
+        // rows of the isotropic Del (ep_create checks it) give exact zeros
 #pragma unroll
         for (int i = 0; i < NS; i++) {
             T acc = T(0);
+            if (i < D) {
 #pragma unroll
-            for (int j = 0; j < D; j++) acc += T(M.Del[i + NS * j]) * ep[j];
+                for (int j = 0; j < D; j++) acc += T(M.Del[i + NS * j]) * ep[j];
+            }
             sp[i] = acc;
         }
         const bool plast = c->with_plast && M.plast != 0;

@@ -1,6 +1,7 @@
 """Per-kernel timing with HIP events (ep_profile, ep_profile_kernel,
 ep_kernel_times): every kernel or the launches of one kernel are timed, and
-timing changes nothing in the state."""
+timing changes nothing in the state.  Also ep_create's check that finite
+strain gets an isotropic Del."""
 from __future__ import annotations
 
 import pytest
@@ -41,3 +42,20 @@ def test_profile_every_kernel_then_one_kernel():
     # the order of the atomic sums)
     assert relerr(st.mpts.x, plain.mpts.x) < 1e-13
     assert relerr(st.mpts.s.tau, plain.mpts.s.tau) < 1e-10
+
+
+def test_finite_strain_rejects_an_anisotropic_del():
+    """The finite-strain update runs in the principal frame of the trial b,
+    which needs the reference's isotropic Del; ep_create says so for any
+    other (EP_E_CONFIG) and accepts the isotropic one."""
+    import numpy as np
+    ic, cfg = slump(3)
+    cm = clone(ic.cmpr)
+    Del = np.array(cm.Del, dtype=np.float64, order="F")
+    Del[3, 3] *= 1.5                        # shear modulus off (a - b)/2
+    cm.Del = Del
+    with pytest.raises(hip.EpError) as e:
+        hip.Device(ic.mesh, ic.mpts, cm, cfg.instr)
+    assert e.value.code == -2 and "isotropic" in str(e.value)
+    dev = hip.Device(ic.mesh, ic.mpts, ic.cmpr, cfg.instr)
+    dev.close()
