@@ -75,7 +75,8 @@ typedef struct ep_config {
     double damping;       /* instr.fwrk.damping */
     double Kc, Gc, Hp;    /* cmpr.Kc, cmpr.Gc, cmpr.Hp */
     double c;             /* cmpr.c, elastic wave speed for get_dt */
-    double Del[36];       /* cmpr.Del, nstr x nstr column-major */
+    double Del[36];       /* cmpr.Del, nstr x nstr column-major; with finite deformation it must be
+                           * the isotropic form setup_cmpr builds (ep_create: EP_E_CONFIG otherwise) */
     double surrogate_tol; /* EP_FP32_SURROGATE: yield-residual tolerance */
     int32_t domain;       /* GIMP domain update, an ep_domain value */
     int32_t nonlocal;     /* instr.nonloc.status: non-local plastic strain (nonlocal.jl:18-47) */
