@@ -190,6 +190,12 @@ extern "C" int ep_create(const ep_config *cfg, ep_ctx **out) {
     if (cfg->precision == EP_FP32_SURROGATE &&
         (cfg->dim != 3 || cfg->deform != EP_FINITE || cfg->plast != EP_DP))
         return fail(EP_E_CONFIG, "the learned return needs 3D finite deformation with Drucker-Prager");
+    if (cfg->nmp <= 0) return fail(EP_E_ARG, "nmp must be positive");
+    if (cfg->nmp > 0x7fffffffL) return fail(EP_E_ARG, "nmp exceeds 2^31 - 1");
+    for (int i = 0; i < cfg->dim; i++) {
+        if (cfg->nel[i] < 1 || cfg->nno[i] < 2 || cfg->nno[i] > (1 << 24) || !(cfg->h[i] > 0.0))
+            return fail(EP_E_ARG, "bad mesh extent");
+    }
     if (cfg->deform == EP_FINITE) {
         // the finite-strain update works in the principal frame of the trial
         // b, which needs an isotropic Del (the reference's, from E and ν):
@@ -209,12 +215,6 @@ extern "C" int ep_create(const ep_config *cfg, ep_ctx **out) {
                 if (zero ? v != 0.0 : !(std::fabs(v - want) <= tol)) { iso = false; break; }
             }
         if (!iso) return fail(EP_E_CONFIG, "finite deformation needs an isotropic Del (the principal-frame update)");
-    }
-    if (cfg->nmp <= 0) return fail(EP_E_ARG, "nmp must be positive");
-    if (cfg->nmp > 0x7fffffffL) return fail(EP_E_ARG, "nmp exceeds 2^31 - 1");
-    for (int i = 0; i < cfg->dim; i++) {
-        if (cfg->nel[i] < 1 || cfg->nno[i] < 2 || cfg->nno[i] > (1 << 24) || !(cfg->h[i] > 0.0))
-            return fail(EP_E_ARG, "bad mesh extent");
     }
     ep_ctx *c = new ep_ctx();
     c->cfg = *cfg;

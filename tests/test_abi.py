@@ -65,3 +65,14 @@ def test_config_errors_without_gpu():
     cfg.plast, cfg.nmp = 1, 0
     assert lib.ep_create(C.byref(cfg), C.byref(h)) == -1
     assert lib.ep_create(None, C.byref(h)) == -1
+    # finite deformation with a Del that is not the isotropic form
+    cfg.nmp = 10
+    for i in range(2):
+        cfg.nel[i], cfg.nno[i], cfg.h[i] = 4, 5, 1.0
+    E, nu = 1.0e6, 0.3
+    f = E / ((1 + nu) * (1 - 2 * nu))
+    Del = [f * (1 - nu), f * nu, 0.0, f * nu, f * (1 - nu), 0.0, 0.0, 0.0, 1.5 * f * (1 - 2 * nu) / 2]
+    for k, v in enumerate(Del):
+        cfg.Del[k] = v
+    assert lib.ep_create(C.byref(cfg), C.byref(h)) == -2
+    assert b"isotropic" in lib.ep_last_error()
