@@ -147,26 +147,27 @@ __device__ __forceinline__ bool weights(const Geom &g, const double *xp, const T
         if (B == 0 && base >= g.t3lo[i] && base + 3 <= g.t3hi[i]) {
             // Four interior nodes: when every ξ lies where bsmpm's first
             // matching branch is the expected piece (k = 0: (1, 2], 1: (0, 1],
-            // 2: (-1, 0], 3: [-2, -1]), evaluate those pieces as bspline()
-            // does, without its branches.
+            // 2: (-1, 0], 3: [-2, -1]), those pieces are the uniform cubic
+            // B-spline's, written in u = ξ₁ and t = -ξ₂ (= 1 - u up to
+            // rounding): (2-ξ₀)³/6 = t³/6, 2/3 - u² + u³/2, 2/3 - t² + t³/2,
+            // (2+ξ₃)³/6 = u³/6, and their derivatives -t²/2, 1.5u² - 2u,
+            // 2t - 1.5t², u²/2 (bsmpm.jl:1-62 expanded; equal up to rounding)
             T xi[4];
 #pragma unroll
             for (int k = 0; k < 4; k++) xi[k] = T((xp[i] - g.X[i][base + k]) * g.ih[i]);
             if (xi[0] > T(1) && xi[0] <= T(2) && xi[1] > T(0) && xi[1] <= T(1) && xi[2] > T(-1) && xi[2] <= T(0) &&
                 xi[3] >= T(-2) && xi[3] <= T(-1)) {
-                const T c16 = T(1.0 / 6.0), c43 = T(4.0 / 3.0), c23 = T(2.0 / 3.0);
-                T x2 = xi[0] * xi[0], x3 = x2 * xi[0];
-                w.w[i][0] = -c16 * x3 + x2 - T(2) * xi[0] + c43;
-                w.dw[i][0] = (T(-0.5) * x2 + T(2) * xi[0] - T(2)) * ih;
-                x2 = xi[1] * xi[1]; x3 = x2 * xi[1];
-                w.w[i][1] = T(0.5) * x3 - x2 + c23;
-                w.dw[i][1] = (T(1.5) * x2 - T(2) * xi[1]) * ih;
-                x2 = xi[2] * xi[2]; x3 = x2 * xi[2];
-                w.w[i][2] = T(-0.5) * x3 - x2 + c23;
-                w.dw[i][2] = (T(-1.5) * x2 - T(2) * xi[2]) * ih;
-                x2 = xi[3] * xi[3]; x3 = x2 * xi[3];
-                w.w[i][3] = c16 * x3 + x2 + T(2) * xi[3] + c43;
-                w.dw[i][3] = (T(0.5) * x2 + T(2) * xi[3] + T(2)) * ih;
+                const T c16 = T(1.0 / 6.0), c23 = T(2.0 / 3.0);
+                const T u = xi[1], t = -xi[2];
+                const T u2 = u * u, u3 = u2 * u, t2 = t * t, t3 = t2 * t;
+                w.w[i][0] = c16 * t3;
+                w.dw[i][0] = T(-0.5) * t2 * ih;
+                w.w[i][1] = T(0.5) * u3 - u2 + c23;
+                w.dw[i][1] = (T(1.5) * u2 - T(2) * u) * ih;
+                w.w[i][2] = T(0.5) * t3 - t2 + c23;
+                w.dw[i][2] = (T(2) * t - T(1.5) * t2) * ih;
+                w.w[i][3] = c16 * u3;
+                w.dw[i][3] = T(0.5) * u2 * ih;
                 w.ok |= 0xFu << (4 * i);
                 continue;
             }
